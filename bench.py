@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the FWI3D wave-propagation hot path on one MI355X.
+
+Workload (one "step"): a single-shot forward modeling pass plus the RTM
+backward pass (source-wavefield reconstruction + receiver back-propagation +
+cross-correlation imaging + illumination) on the 401 x 401 x 201 interior
+grid at 10 m with a 50-cell random boundary (501 x 501 x 301 padded), 3000
+samples at dt = 0.8 ms (3074 steps per pass), 201 x 201 receivers.  Inputs
+(model, wavelet, geometry, the data injected by the backward pass) are
+resident in HBM before timing starts; data are synthetic (the seeded
+gradient/dipping-layer/anomaly model of fwi3d_amd.configs).
+
+Prints one JSON line.  `value` is stencil throughput in Gcell-updates/s over
+the whole RTM shot (forward: 1 field, backward: 2 fields per cell and step).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E vendor peak
+BYTES_PER_UPDATE = 16   # read cur, old, vp; write new
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, vp_pad, seconds):
+    """Oracle (OpenMP, reference arithmetic) on a bounded sample of the same
+    workload: a few forward steps and a few backward (RTM) steps on the full
+    padded grid, scaled to Gcell-updates/s."""
+    import oracle as O
+    O.build()
+    threads = O.num_threads()
+    cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
+    # forward steps around the source-injection window
+    t0 = time.perf_counter()
+    P, Pold, _ = O.forward(vp_pad, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src, w.rec[:1], n_begin=0,
+                           n_end=2, record=False)
+    t_fwd2 = time.perf_counter() - t0
+    nf = max(2, int(0.5 * seconds / max(t_fwd2 / 2, 1e-3)))
+    t0 = time.perf_counter()
+    P, Pold, _ = O.forward(vp_pad, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src, w.rec[:1], n_begin=2,
+                           n_end=2 + nf, P=P, Pold=Pold, record=False)
+    t_fwd = time.perf_counter() - t0
+    data = np.zeros((1, w.nt), np.float32)
+    nbk = max(1, nf // 2)
+    t0 = time.perf_counter()
+    O.backward(vp_pad, w.dh, w.dt, w.nt, data, w.rec[:1], P, Pold, b_begin=0, b_end=nbk, tlag=w.tlag)
+    t_bwd = time.perf_counter() - t0
+    updates = cells * (nf + 2 * nbk)
+    rate = updates / (t_fwd + t_bwd) / 1e9
+    return {"value": round(rate, 4), "unit": "Gcell-upd/s", "cores": threads, "kind": "port",
+            "sample": "%d forward + %d RTM backward steps on the full 501x501x301 padded grid (%.1f s)"
+                      % (nf, nbk, t_fwd + t_bwd)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--arith", choices=["reference", "fp32"], default="reference")
+    ap.add_argument("--graph", action="store_true", help="capture each pass in a hipGraph")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X: multi-GPU shot sharding is not part of this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    from fwi3d_amd import configs, lib
+
+    log("building workload gradient_401 ...")
+    w = configs.gradient_401()
+    log("grid %dx%dx%d padded, nb=%d, nt=%d, tlag=%d, nrec=%d" % (w.nxx, w.nyy, w.nzz, w.nb, w.nt, w.tlag,
+                                                                  len(w.rec)))
+    ctx = lib.Context(w.nxx, w.nyy, w.nzz, w.nb, w.dh, w.dt, w.nt, w.tlag)
+    ctx.set_arith(lib.ARITH_FP32 if args.arith == "fp32" else lib.ARITH_REFERENCE)
+    ctx.set_model_interior(w.interior)
+    ctx.random_boundary(w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+    ctx.set_wavelet(w.wavelet)
+    ctx.set_source(*w.src)
+    ctx.set_receivers(w.rec)
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        ctx.autotune(lib.KIND_FWD_RBC, 4)
+        ctx.autotune(lib.KIND_BWD_RTM, 4)
+        log("autotune %.1f s: forward %s, rtm %s" % (time.perf_counter() - t0, ctx.tuning(0), ctx.tuning(2)))
+    ctx.set_graph(args.graph)
+    ctx.set_kernel_timing(not args.graph)
+
+    # synthetic observed data: the forward pass's own recording
+    ctx.forward(lib.BOUNDARY_RBC)
+    ctx.set_backward_data(ctx.seismogram())
+    log("data ready")
+
+    def step():
+        ctx.forward(lib.BOUNDARY_RBC)
+        f = ctx.last_pass_timing() + ctx.last_kernel_timing()
+        ctx.rtm_backward()
+        b = ctx.last_pass_timing() + ctx.last_kernel_timing()
+        return f, b
+
+    for i in range(args.warmup):
+        step()
+        log("warmup %d done" % i)
+    ctx.zero_volume(lib.VOL_IMAGE)
+    ctx.zero_volume(lib.VOL_ILLUM)
+    ctx.sync()
+    t0 = time.perf_counter()
+    fwd_k = bwd_k = fwd_pass = bwd_pass = 0.0
+    nlaunch_f = nlaunch_b = 0
+    for i in range(args.steps):
+        (fp, _, fk, fl), (bp, _, bk, bl) = step()
+        fwd_pass += fp
+        bwd_pass += bp
+        fwd_k += fk
+        bwd_k += bk
+        nlaunch_f += fl
+        nlaunch_b += bl
+        log("step %d: forward %.1f ms, rtm backward %.1f ms" % (i, fp, bp))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+
+    cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
+    N = w.steps
+    updates_per_step = cells * N * 3
+    sec_per_step = wall / args.steps
+    value = updates_per_step / sec_per_step / 1e9
+
+    roofline = None
+    extra = {}
+    if nlaunch_b and nlaunch_f:
+        bwd_avg_s = bwd_k / nlaunch_b / 1e3
+        fwd_avg_s = fwd_k / nlaunch_f / 1e3
+        bwd_bytes = 2 * BYTES_PER_UPDATE * cells
+        fwd_bytes = BYTES_PER_UPDATE * cells
+        ach = bwd_bytes / bwd_avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "step_kernel<BWD_RTM> (fused reconstruction + receiver propagation + imaging)",
+                    "avg_launch_us": round(bwd_avg_s * 1e6, 2), "algorithmic_bytes_per_launch": bwd_bytes}
+        extra["forward_kernel"] = {"avg_launch_us": round(fwd_avg_s * 1e6, 2),
+                                   "achieved_GBs": round(fwd_bytes / fwd_avg_s / 1e9, 1),
+                                   "frac": round(fwd_bytes / fwd_avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    pct_hbm = value * BYTES_PER_UPDATE / HBM_PEAK_GBS
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        import oracle as O
+        vp_pad = O.random_boundary(O.expand_boundary(w.interior, w.nb), w.nb, w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+        ctx_vp = ctx.get_volume(lib.VOL_MODEL)
+        if not np.array_equal(ctx_vp, vp_pad):
+            log("warning: device model differs from the oracle's")
+        ctx.close()
+        cpu = cpu_baseline(w, vp_pad, args.cpu_seconds)
+    else:
+        ctx.close()
+
+    out = {
+        "metric": "RTM shots/h batched vs per-shot; stencil Gcell-upd/s %HBM; FWI time-to-misfit",
+        "value": round(value, 3),
+        "unit": "Gcell-upd/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_per_step * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (per-axis FP64 derivative)" if args.arith == "reference" else "f32",
+        "data": "synthetic",
+        "config": {"workload": "single-shot forward + RTM, 401x401x201 @10 m, nb=50 random boundary, "
+                               "nt=3000, dt=0.8 ms, 201x201 receivers",
+                   "padded_grid": [w.nxx, w.nyy, w.nzz], "steps_per_pass": N, "arith": args.arith,
+                   "graph": args.graph},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "rtm_shots_per_hour_per_shot_path": round(3600.0 / sec_per_step, 1),
+        "pct_hbm_at_16B_per_update": round(pct_hbm, 4),
+        "forward_pass_ms": round(fwd_pass / args.steps, 2),
+        "rtm_backward_pass_ms": round(bwd_pass / args.steps, 2),
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,86 @@
+"""Build script for the native parts of fwi3d_amd.
+
+    python -m fwi3d_amd.build            # HIP library + host library + drivers
+
+Outputs (in-tree, git-ignored):
+    fwi3d_amd/libfwi3d_hip.so     hand-written gfx950 kernels + C ABI (include/fwi3d.h)
+    fwi3d_amd/libfwi3d_host.so    C++ Modeling / Migration / Inversion classes
+    bin/modeling_main, bin/migration_main, bin/inversion_main   drivers
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+HIP_LIB = os.path.join(PKG, "libfwi3d_hip.so")
+HOST_LIB = os.path.join(PKG, "libfwi3d_host.so")
+BIN = os.path.join(ROOT, "bin")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# -ffp-contract=off: one rounding per written operation (the arithmetic
+# contract shared with the CPU oracle).  Packed-FP32 instructions are
+# disabled for all HIP code.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+HIP_SOURCES = ["abi.hip", "kernels_step.hip", "kernels_aux.hip"]
+
+CXX = shutil.which("g++") or "g++"
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-ffp-contract=off",
+              "-march=x86-64-v3"]
+HOST_SOURCES = ["admin.cpp", "geometry.cpp", "modeling.cpp", "migration.cpp", "inversion.cpp", "capi.cpp"]
+DRIVERS = ["modeling_main", "migration_main", "inversion_main"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return out
+
+
+def build_hip():
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", HIP_LIB + ".tmp"])
+    os.replace(HIP_LIB + ".tmp", HIP_LIB)
+    for o in objs:
+        os.remove(o)
+    return HIP_LIB
+
+
+def build_host():
+    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES if os.path.exists(os.path.join(HOST, s))]
+    if not srcs:
+        return None
+    _run([CXX] + HOST_FLAGS + ["-shared"] + srcs + ["-o", HOST_LIB + ".tmp",
+         "-L" + PKG, "-lfwi3d_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for d in DRIVERS:
+        src = os.path.join(HOST, d + ".cpp")
+        if os.path.exists(src):
+            _run([CXX] + HOST_FLAGS + [src, "-o", os.path.join(BIN, d), "-L" + PKG, "-lfwi3d_host",
+                 "-lfwi3d_hip", "-Wl,-rpath,$ORIGIN/../fwi3d_amd"])
+    return HOST_LIB
+
+
+def build_all():
+    build_hip()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", HIP_LIB)

@@ -1,0 +1,114 @@
+// common.hpp -- shared definitions of the gfx950 wave-propagation library.
+//
+// Device volume layout ("x-fast"):   e(x, y, z) = x + y*px + z*pp
+//   x = reference j (model_x), y = reference k (model_y), z = reference i
+//   (depth); px = nxx rounded up to 64 floats (256-byte rows), pp = px*nyy.
+// The step kernels tile the x-y plane and march along z, so every HBM row
+// access is a 256-byte coalesced wave load.  Host volumes keep the
+// reference layout (i fastest) and are transposed on upload/download.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace fwi {
+
+// src/modeling/modeling.cuh:9-13 -- the reference's 8th-order second
+// derivative coefficients, the same truncated double literals.
+constexpr double FDM1 = 0.001785714285;
+constexpr double FDM2 = 0.025396825396;
+constexpr double FDM3 = 0.200000000000;
+constexpr double FDM4 = 1.600000000000;
+constexpr double FDM5 = 2.847222222222;
+
+// Largest stable Courant number v*dt/dh of the 3-D 8th-order leapfrog
+// scheme: sqrt(4 / (3 * 6.501587...)), with 6.501587 = |symbol at Nyquist|.
+constexpr double CFL_MAX = 0.452856;
+
+enum Mode { FWD_RBC = 0, FWD_ABC = 1, BWD_RTM = 2, BWD_FWI = 3 };
+enum Arith { ARITH_REF = 0, ARITH_FP32 = 1 };
+
+struct Layout {
+    int nx, ny, nz;  // padded sizes (x = reference nxx, y = nyy, z = nzz)
+    int px;          // row pitch in floats
+    long pp;         // plane pitch in floats
+    long vol;        // allocation in floats
+    __host__ __device__ long at(int x, int y, int z) const { return (long)x + (long)y * px + (long)z * pp; }
+};
+
+inline Layout make_layout(int nxx, int nyy, int nzz) {
+    Layout L;
+    L.nx = nxx;
+    L.ny = nyy;
+    L.nz = nzz;
+    L.px = (nxx + 63) / 64 * 64;
+    L.pp = (long)L.px * nyy;
+    L.vol = L.pp * nzz;
+    return L;
+}
+
+// Launch configuration of a step kernel.
+struct Tuning {
+    int by = 4;       // waves (rows of 64 lanes) per block
+    int rpt = 2;      // tile rows per thread (register blocking in y)
+    int zchunks = 1;  // blocks along z per (x, y) tile column
+};
+
+// Arguments of one step launch.  The fields of shot s live at base + s*vol.
+struct StepArgs {
+    Layout L;
+    long sstride;          // floats between consecutive shots' volumes
+    int nshots;            // shots in this launch
+    int ntx, nty, zlen, nzc;
+    int nitems;            // ntx * nty * nzc * shot groups
+    const float* vp;       // per shot
+    const float* cur;      // field read by the stencil: A (forward) / So (backward)
+    float* old;            // in place: previous -> next
+    const float* rcur;     // backward: receiver field R
+    float* rold;           // backward: in place Ro -> next
+    float* image;          // RTM image or FWI gradient (one per shot group)
+    float* illum;          // illumination (one per shot group)
+    long istride;          // floats between shot groups' image/illum
+    float idh2, dt2, dt;
+    // forward source injection of the next step (owner thread adds inj)
+    const long* src_off;   // per shot, device offset of the source cell (forward only)
+    float inj;             // idh3 * wavelet[n+1], 0 when n+1 >= nt
+    // ABC
+    const float* b1d;
+    int nb;
+    // forward receiver gather into a time-major row: seis[r] = cur[rec] * damp
+    const long* rec_off;
+    const float* rec_damp;
+    float* seis_row;       // NULL: no recording this step
+    const int* rec_start;  // per shot: receivers [rec_start[s], rec_start[s+1])
+    long seis_sstride;     // floats between shots' seismogram rows
+};
+
+// Host-side launchers (kernels_step.hip).
+hipError_t launch_step(int mode, int arith, const Tuning& t, int sb, const StepArgs& a, hipStream_t st);
+bool step_config_supported(int mode, int arith, const Tuning& t, int sb);
+void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
+
+// Host-side launchers (kernels_aux.hip).  "host layout" buffers here are
+// device buffers holding data in the reference layout (staging).
+hipError_t launch_to_device(const float* host_layout, float* dev, const Layout& L, hipStream_t st);
+hipError_t launch_to_host(const float* dev, float* host_layout, const Layout& L, int crop, hipStream_t st);
+hipError_t launch_expand(const float* interior_host_layout, float* dev, const Layout& L, int nb, hipStream_t st);
+hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t seed, int shot, float amp,
+                                  float vclamp, hipStream_t st);
+hipError_t launch_add_point(float* dev, long off, float value, hipStream_t st);
+hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec,
+                         int ncells, const float* data_row, float idh3, hipStream_t st);
+hipError_t launch_export_image(const float* image, const float* illum, float* out, const Layout& L, int nb,
+                               float idh2, int zfirst, hipStream_t st);
+hipError_t launch_scale_by_inverse(float* v, long n, const double* gmax, hipStream_t st);
+hipError_t launch_absmax(const float* v, long n, double* partial, double* result, hipStream_t st);
+hipError_t launch_residual(const float* obs, float* cal, long n, double* partial, double* result, hipStream_t st);
+hipError_t launch_stats(const float* v, const Layout& L, double* partial, double* result, hipStream_t st);
+hipError_t launch_transpose2d(const float* src, float* dst, int rows, int cols, hipStream_t st);
+
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace fwi

@@ -1,0 +1,342 @@
+// kernels_aux.hip -- layout transposes, model set-up, injection and the
+// post-processing / FP64 reduction kernels around the time-step kernels.
+#include "common.hpp"
+
+namespace fwi {
+
+namespace {
+
+constexpr int TT = 32;  // transpose tile
+
+// Reference layout (i fastest) -> device layout (x fastest), full volume.
+// One k-slab per blockIdx.z is a (nxx x nzz) matrix transposed via LDS.
+__global__ void __launch_bounds__(256) to_device_kernel(const float* __restrict__ src, float* __restrict__ dev,
+                                                        Layout L) {
+    __shared__ float tile[TT][TT + 1];
+    const int i0 = blockIdx.x * TT, j0 = blockIdx.y * TT, k = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const long slab = (long)k * L.nx * L.nz;
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int j = j0 + ty + m, i = i0 + tx;
+        if (j < L.nx && i < L.nz) tile[ty + m][tx] = src[slab + (long)j * L.nz + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int i = i0 + ty + m, j = j0 + tx;
+        if (j < L.nx && i < L.nz) dev[L.at(j, k, i)] = tile[tx][ty + m];
+    }
+}
+
+// Device layout -> reference layout, optionally cropping c cells per side.
+__global__ void __launch_bounds__(256) to_host_kernel(const float* __restrict__ dev, float* __restrict__ dst,
+                                                      Layout L, int c) {
+    __shared__ float tile[TT][TT + 1];
+    const int nzo = L.nz - 2 * c, nxo = L.nx - 2 * c;
+    const int i0 = blockIdx.x * TT, j0 = blockIdx.y * TT, k = blockIdx.z;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int i = i0 + ty + m, j = j0 + tx;
+        if (j < nxo && i < nzo) tile[ty + m][tx] = dev[L.at(j + c, k + c, i + c)];
+    }
+    __syncthreads();
+    const long slab = (long)k * nxo * nzo;
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int j = j0 + ty + m, i = i0 + tx;
+        if (j < nxo && i < nzo) dst[slab + (long)j * nzo + i] = tile[tx][ty + m];
+    }
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Modeling::expand_boundary (modeling.cu:181-228): the reference's three
+// face passes amount to replicating the nearest interior cell.
+__global__ void __launch_bounds__(256) expand_kernel(const float* __restrict__ interior, float* __restrict__ dev,
+                                                     Layout L, int nb) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= L.nx || y >= L.ny) return;
+    const int nx = L.nx - 2 * nb, ny = L.ny - 2 * nb, nz = L.nz - 2 * nb;
+    const int i = clampi(z - nb, 0, nz - 1), j = clampi(x - nb, 0, nx - 1), k = clampi(y - nb, 0, ny - 1);
+    dev[L.at(x, y, z)] = interior[(long)i + (long)j * nz + (long)k * nx * nz];
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Seeded random boundary: v = v_face*(1 + amp*ramp*u), u ~ U[-1,1) from a
+// counter hash of (seed, shot, reference cell index), ramp = 1 - d/nb
+// (random_boundary_bg's ramp, modeling.cu:637-662), max over boundary axes.
+__global__ void __launch_bounds__(256) random_boundary_kernel(float* __restrict__ dev, Layout L, int nb, uint64_t seed,
+                                                              int shot, float amp, float vclamp) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= L.nx || y >= L.ny) return;
+    const bool in_z = z >= nb && z < L.nz - nb, in_x = x >= nb && x < L.nx - nb, in_y = y >= nb && y < L.ny - nb;
+    if (in_z && in_x && in_y) return;
+    const float v = dev[L.at(clampi(x, nb, L.nx - nb - 1), clampi(y, nb, L.ny - nb - 1), clampi(z, nb, L.nz - nb - 1))];
+    float ramp = 0.0f;
+    if (!in_x) { const int d = x < nb ? x : L.nx - x - 1; ramp = fmaxf(ramp, 1.0f - (float)d / (float)nb); }
+    if (!in_y) { const int d = y < nb ? y : L.ny - y - 1; ramp = fmaxf(ramp, 1.0f - (float)d / (float)nb); }
+    if (!in_z) { const int d = z < nb ? z : L.nz - z - 1; ramp = fmaxf(ramp, 1.0f - (float)d / (float)nb); }
+    const long cell = (long)z + (long)x * L.nz + (long)y * L.nx * L.nz;
+    const uint64_t key = (seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)shot * 0xD1B54A32D192ED03ull) ^
+                         (uint64_t)cell;
+    const int u24 = (int)(splitmix64(key) >> 40);
+    const float u = (float)(u24 - 8388608) * (1.0f / 8388608.0f);
+    float t = amp * ramp;
+    t = t * u;
+    const float f = 1.0f + t;
+    float val = v * f;
+    if (vclamp > 0.0f && val > vclamp) val = vclamp;
+    dev[L.at(x, y, z)] = val;
+}
+
+__global__ void add_point_kernel(float* dev, long off, float value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) dev[off] = dev[off] + value;
+}
+
+// inject_seismogram / inject_residuo (migration.cu:119-132,
+// inversion.cu:385-398): field[cell] += idh3 * data[r] for every receiver r
+// on the cell, summed in receiver order (deterministic, unlike atomics).
+__global__ void __launch_bounds__(256) inject_kernel(float* __restrict__ field, const long* __restrict__ cell_off,
+                                                     const int* __restrict__ cell_start,
+                                                     const int* __restrict__ cell_rec, int ncells,
+                                                     const float* __restrict__ data_row, float idh3) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const long off = cell_off[c];
+    float v = field[off];
+    for (int q = cell_start[c]; q < cell_start[c + 1]; ++q) v = v + idh3 * data_row[cell_rec[q]];
+    field[off] = v;
+}
+
+// Migration::export_seismic (migration.cu:90-117) into the reference
+// interior layout: I = image/illum, out = -(I[i-1] - 2I[i] + I[i+1])*idh2.
+__global__ void __launch_bounds__(256) export_image_kernel(const float* __restrict__ image,
+                                                           const float* __restrict__ illum, float* __restrict__ out,
+                                                           Layout L, int nb, float idh2, int zfirst) {
+    const int nz = L.nz - 2 * nb, nx = L.nx - 2 * nb, ny = L.ny - 2 * nb;
+    const long n = (long)nx * ny * nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(q / ((long)nx * nz));
+        const int j = (int)((q - (long)k * nx * nz) / nz);
+        const int i = (int)(q - (long)j * nz - (long)k * nx * nz);
+        float v = 0.0f;
+        if (i > zfirst && i < nz - 1) {
+            const long c = L.at(j + nb, k + nb, i + nb);
+            const float a = image[c - L.pp] / illum[c - L.pp];
+            const float b = image[c] / illum[c];
+            const float d = image[c + L.pp] / illum[c + L.pp];
+            v = -1.0f * (a - 2.0f * b + d) * idh2;
+        }
+        out[q] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) scale_kernel(float* __restrict__ v, long n, const double* __restrict__ gmax) {
+    const float s = 1.0f / (float)gmax[0];
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        v[q] = v[q] * s;
+}
+
+// ---- FP64 two-pass reductions (fixed order: deterministic) ---------------
+
+template <int OP>
+__device__ __forceinline__ double combine(double a, double b) {
+    if constexpr (OP == 0) return a + b;        // sum
+    else if constexpr (OP == 1) return fmax(a, b);  // max
+    else return fmin(a, b);                      // min
+}
+
+template <int OP>
+__device__ double block_reduce(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = combine<OP>(sh[t], sh[t + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// |v| max over a contiguous array (gradient normalisation, inversion.cu:152-160).
+__global__ void __launch_bounds__(256) absmax_partial(const float* __restrict__ v, long n, double* partial) {
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) m = fmax(m, (double)fabsf(v[q]));
+    const double r = block_reduce<1>(m, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// residual r = obs - cal in place over cal, and per-block sum of (double)r^2
+// (Inversion::set_adjoint_source, inversion.cu:190-201).
+__global__ void __launch_bounds__(256) residual_partial(const float* __restrict__ obs, float* __restrict__ cal, long n,
+                                                        double* partial) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const float r = obs[q] - cal[q];
+        cal[q] = r;
+        s += (double)r * (double)r;
+    }
+    const double r = block_reduce<0>(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// max, min and non-finite count over the cells of a padded device volume.
+__global__ void __launch_bounds__(256) stats_partial(const float* __restrict__ v, Layout L, double* partial) {
+    __shared__ double sh[256];
+    double mx = -INFINITY, mn = INFINITY, bad = 0.0;
+    const long n = (long)L.nx * L.ny * L.nz;
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) {
+        const int x = (int)(q % L.nx);
+        const long r = q / L.nx;
+        const int y = (int)(r % L.ny);
+        const int z = (int)(r / L.ny);
+        const float f = v[L.at(x, y, z)];
+        if (!isfinite(f)) {
+            bad += 1.0;
+        } else {
+            mx = fmax(mx, (double)f);
+            mn = fmin(mn, (double)f);
+        }
+    }
+    const double a = block_reduce<1>(mx, sh);
+    __syncthreads();
+    const double b = block_reduce<2>(mn, sh);
+    __syncthreads();
+    const double c = block_reduce<0>(bad, sh);
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = a;
+        partial[3 * blockIdx.x + 1] = b;
+        partial[3 * blockIdx.x + 2] = c;
+    }
+}
+
+// Second pass: one block folds `nparts` partials (`nval` values each, op
+// codes 0 sum / 1 max / 2 min) in a fixed order.
+__global__ void __launch_bounds__(256) finish_kernel(const double* __restrict__ partial, int nparts, int nval,
+                                                     int op0, int op1, int op2, double* result) {
+    __shared__ double sh[256];
+    for (int v = 0; v < nval; ++v) {
+        const int op = v == 0 ? op0 : (v == 1 ? op1 : op2);
+        double acc = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
+        for (int q = threadIdx.x; q < nparts; q += 256) {
+            const double p = partial[(long)q * nval + v];
+            acc = op == 0 ? acc + p : (op == 1 ? fmax(acc, p) : fmin(acc, p));
+        }
+        double r;
+        if (op == 0) r = block_reduce<0>(acc, sh);
+        else if (op == 1) r = block_reduce<1>(acc, sh);
+        else r = block_reduce<2>(acc, sh);
+        if (threadIdx.x == 0) result[v] = r;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) transpose2d_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                          int rows, int cols) {
+    __shared__ float tile[TT][TT + 1];
+    const int c0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int r = r0 + ty + m, c = c0 + tx;
+        if (r < rows && c < cols) tile[ty + m][tx] = src[(long)r * cols + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < TT; m += 8) {
+        const int c = c0 + ty + m, r = r0 + tx;
+        if (r < rows && c < cols) dst[(long)c * rows + r] = tile[tx][ty + m];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_to_device(const float* src, float* dev, const Layout& L, hipStream_t st) {
+    dim3 grid((L.nz + TT - 1) / TT, (L.nx + TT - 1) / TT, L.ny);
+    to_device_kernel<<<grid, dim3(TT, 8), 0, st>>>(src, dev, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_host(const float* dev, float* dst, const Layout& L, int crop, hipStream_t st) {
+    dim3 grid((L.nz - 2 * crop + TT - 1) / TT, (L.nx - 2 * crop + TT - 1) / TT, L.ny - 2 * crop);
+    to_host_kernel<<<grid, dim3(TT, 8), 0, st>>>(dev, dst, L, crop);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const float* interior, float* dev, const Layout& L, int nb, hipStream_t st) {
+    dim3 grid((L.nx + 63) / 64, (L.ny + 3) / 4, L.nz);
+    expand_kernel<<<grid, dim3(64, 4), 0, st>>>(interior, dev, L, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t seed, int shot, float amp,
+                                  float vclamp, hipStream_t st) {
+    dim3 grid((L.nx + 63) / 64, (L.ny + 3) / 4, L.nz);
+    random_boundary_kernel<<<grid, dim3(64, 4), 0, st>>>(dev, L, nb, seed, shot, amp, vclamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_point(float* dev, long off, float value, hipStream_t st) {
+    add_point_kernel<<<1, 64, 0, st>>>(dev, off, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec, int ncells,
+                         const float* data_row, float idh3, hipStream_t st) {
+    if (ncells <= 0) return hipSuccess;
+    inject_kernel<<<(ncells + 255) / 256, 256, 0, st>>>(field, cell_off, cell_start, cell_rec, ncells, data_row,
+                                                          idh3);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_image(const float* image, const float* illum, float* out, const Layout& L, int nb,
+                               float idh2, int zfirst, hipStream_t st) {
+    export_image_kernel<<<2048, 256, 0, st>>>(image, illum, out, L, nb, idh2, zfirst);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_by_inverse(float* v, long n, const double* gmax, hipStream_t st) {
+    scale_kernel<<<2048, 256, 0, st>>>(v, n, gmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float* v, long n, double* partial, double* result, hipStream_t st) {
+    absmax_partial<<<kReduceBlocks, 256, 0, st>>>(v, n, partial);
+    finish_kernel<<<1, 256, 0, st>>>(partial, kReduceBlocks, 1, 1, 0, 0, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const float* obs, float* cal, long n, double* partial, double* result, hipStream_t st) {
+    residual_partial<<<kReduceBlocks, 256, 0, st>>>(obs, cal, n, partial);
+    finish_kernel<<<1, 256, 0, st>>>(partial, kReduceBlocks, 1, 0, 0, 0, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(const float* v, const Layout& L, double* partial, double* result, hipStream_t st) {
+    stats_partial<<<kReduceBlocks, 256, 0, st>>>(v, L, partial);
+    finish_kernel<<<1, 256, 0, st>>>(partial, kReduceBlocks, 3, 1, 2, 0, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose2d(const float* src, float* dst, int rows, int cols, hipStream_t st) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    dim3 grid((cols + TT - 1) / TT, (rows + TT - 1) / TT);
+    transpose2d_kernel<<<grid, dim3(TT, 8), 0, st>>>(src, dst, rows, cols);
+    return hipGetLastError();
+}
+
+}  // namespace fwi

@@ -1,0 +1,199 @@
+/*
+ * fwi3d.h -- C ABI of libfwi3d_hip.so, the MI355X (gfx950) wave-propagation
+ * library behind FWI3D's Modeling / Migration / Inversion classes.
+ *
+ * The reference (phbastosa/FWI3D) has no FFI: its classes launch CUDA kernels
+ * directly.  Every entry point below replaces one of those launch sites or
+ * the host loop around it; the comment on each names the reference code it
+ * stands in for.  Everything is plain C: opaque context, raw pointers, sizes.
+ *
+ * Conventions
+ *  - Every call returns an int status: FWI_OK (0) or a negative FWI_ERR_*.
+ *    fwi_last_error() returns a human-readable message for the last failure
+ *    on a context (or of fwi_create when ctx is NULL).  No C++ exception
+ *    crosses this boundary.
+ *  - Host buffers are owned by the caller.  Calls that take or return host
+ *    buffers are synchronous on return; volumes live in device memory (HBM)
+ *    between calls.
+ *  - Host volumes use the reference layout  c = i + j*nzz + k*nxx*nzz
+ *    (i = z fastest, j = x, k = y; src/modeling/modeling.cu:190).
+ *    Seismograms use the reference file layout  t + r*nt
+ *    (src/modeling/modeling.cu:392).
+ *  - A context is used from one host thread at a time.  Contexts on
+ *    different devices may run concurrently.
+ */
+#ifndef FWI3D_H
+#define FWI3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWI_OK 0
+#define FWI_ERR_ARG (-1)       /* bad argument (NULL pointer, size, enum) */
+#define FWI_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define FWI_ERR_CFL (-3)       /* model violates v*dt/dh <= 0.452856 */
+#define FWI_ERR_STATE (-4)     /* call order: e.g. backward before forward */
+#define FWI_ERR_RANGE (-5)     /* source/receiver outside the updated grid */
+#define FWI_ERR_NONFINITE (-6) /* NaN/Inf found in a model or wavefield */
+#define FWI_ERR_NODEV (-7)     /* no HIP device */
+
+/* boundary treatment of a forward pass */
+#define FWI_BOUNDARY_RBC 0 /* no damping: random-boundary model, reversible */
+#define FWI_BOUNDARY_ABC 1 /* Cerjan damping (modeling.cu:374-380) */
+
+/* arithmetic of the stencil */
+#define FWI_ARITH_REFERENCE 0 /* per-axis derivative summed in FP64, as the
+                                 reference's double FD literals do
+                                 (modeling.cuh:9-13); bit-exact with the oracle */
+#define FWI_ARITH_FP32 1      /* all-FP32 FMA stencil */
+
+/* volumes for fwi_get_volume / fwi_set_volume */
+#define FWI_VOL_MODEL 0     /* vp (padded) */
+#define FWI_VOL_FIELD 1     /* source-side current field (d_P) */
+#define FWI_VOL_FIELD_OLD 2 /* source-side previous field (d_Pold) */
+#define FWI_VOL_RECV 3      /* receiver-side current field (d_Pr) */
+#define FWI_VOL_RECV_OLD 4  /* receiver-side previous field (d_Prold) */
+#define FWI_VOL_IMAGE 5     /* RTM image accumulator (d_image) */
+#define FWI_VOL_ILLUM 6     /* illumination accumulator (d_sumPs) */
+#define FWI_VOL_GRAD 7      /* FWI gradient accumulator (d_gradient_rbc) */
+
+/* Padded grid.  nxx = nx + 2*nb etc. (modeling.cu:146-150). */
+typedef struct fwi_grid {
+    int nxx, nyy, nzz; /* padded samples in x, y, z */
+    int nb;            /* boundary cells per side */
+    float dh;          /* grid spacing [m] */
+    float dt;          /* time step [s] */
+    int nt;            /* recorded samples (time_samples) */
+    int tlag;          /* wavelet lag in steps; a pass runs nt + tlag steps */
+} fwi_grid;
+
+typedef struct fwi_ctx fwi_ctx;
+
+/* Create a context on `device` and allocate the padded model and the two
+ * source-side fields.  Stands in for Modeling::set_properties' device
+ * allocations (modeling.cu:142-179).  Receiver-side fields, image,
+ * illumination and gradient are allocated on first use. */
+int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out);
+int fwi_destroy(fwi_ctx* ctx);
+const char* fwi_last_error(const fwi_ctx* ctx);
+
+/* FWI_ARITH_REFERENCE (default) or FWI_ARITH_FP32. */
+int fwi_set_arith(fwi_ctx* ctx, int arith);
+
+/* Padded velocity model, host layout (cudaMemcpy of d_Vp, modeling.cu:176).
+ * Rejects NaN/Inf and CFL violations. */
+int fwi_set_model(fwi_ctx* ctx, const float* vp_padded);
+
+/* Interior model (nz*nx*ny, host layout) padded on the device by edge
+ * replication (Modeling::expand_boundary, modeling.cu:181-228). */
+int fwi_set_model_interior(fwi_ctx* ctx, const float* vp_interior);
+
+/* Seeded random boundary on the current model, in place on the device:
+ * v = v_face*(1 + amp*ramp*u), u ~ U[-1,1) from a counter hash of
+ * (seed, shot, cell), ramp = 1 - d/nb, clamped to vclamp when vclamp > 0.
+ * Replaces Modeling::set_random_boundary (modeling.cu:558-666), whose curand
+ * and Poisson-disk noise are not reproducible. */
+int fwi_random_boundary(fwi_ctx* ctx, uint64_t seed, int shot, float amp, float vclamp);
+
+/* Source wavelet, nt samples (cudaMemcpy of d_wavelet, modeling.cu:54-56). */
+int fwi_set_wavelet(fwi_ctx* ctx, const float* wavelet);
+
+/* 1-D Cerjan damper b1d[nb] (modeling.cu:96-99); the 2-D and 3-D dampers of
+ * get_boundary_damper (modeling.cu:395-520) are derived from it on the fly
+ * with the same float operations as the reference's b2d/b3d tables. */
+int fwi_set_dampers(fwi_ctx* ctx, const float* b1d);
+
+/* Source and receiver padded indices (Modeling::initialization,
+ * modeling.cu:264-300).  Positions must lie in the updated interior
+ * 4 <= idx < n-4. */
+int fwi_set_source(fwi_ctx* ctx, int sIdx, int sIdy, int sIdz);
+int fwi_set_receivers(fwi_ctx* ctx, int nrec, const int* rIdx, const int* rIdy, const int* rIdz);
+
+/* Full forward pass, nt + tlag steps from zero fields, recording the
+ * seismogram on the device (Modeling::forward_solver with compute_pressure
+ * and compute_seismogram, modeling.cu:309-393). */
+int fwi_forward(fwi_ctx* ctx, int boundary);
+
+/* Steps [n_begin, n_end) of the forward loop from the current fields
+ * (no reset); for step-wise checks. */
+int fwi_forward_steps(fwi_ctx* ctx, int boundary, int n_begin, int n_end);
+
+/* Recorded seismogram, nt*nrec floats, layout t + r*nt (cudaMemcpy of
+ * d_seismogram, modeling.cu:304).  Sample 0 of every trace is 0. */
+int fwi_get_seismogram(fwi_ctx* ctx, float* out);
+
+/* Upload the data injected by the backward pass (nt*nrec, t + r*nt):
+ * Migration::set_seismic_source (migration.cu:83-88). */
+int fwi_set_backward_data(fwi_ctx* ctx, const float* data);
+
+/* Full RTM backward pass (Migration::backward_propagation with
+ * inject_seismogram and cross_correlation, migration.cu:61-81,119-202).
+ * Zeroes the receiver fields, injects the uploaded data time-reversed and
+ * accumulates image and illumination (not reset between shots). */
+int fwi_rtm_backward(fwi_ctx* ctx);
+
+/* Full FWI backward pass (Inversion::backward_propagation with
+ * inject_residuo and build_gradient, inversion.cu:235-253,385-468); uses the
+ * uploaded data (the residual) and accumulates gradient and illumination. */
+int fwi_grad_backward(fwi_ctx* ctx);
+
+/* Backward steps [b_begin, b_end) from the current fields; kind 0 = RTM,
+ * 1 = FWI.  b_begin == 0 zeroes nothing: call fwi_zero_volume first. */
+int fwi_backward_steps(fwi_ctx* ctx, int kind, int b_begin, int b_end);
+
+/* Zero one volume (cudaMemset in the reference, e.g. migration.cu:81-82). */
+int fwi_zero_volume(fwi_ctx* ctx, int which);
+
+/* Copy a padded volume to / from the host (host layout). */
+int fwi_get_volume(fwi_ctx* ctx, int which, float* out_padded);
+int fwi_set_volume(fwi_ctx* ctx, int which, const float* in_padded);
+
+/* Migration::export_seismic arithmetic on the device (migration.cu:90-117):
+ * crop, image/illumination, minus z second difference; nz*nx*ny out. */
+int fwi_export_image(fwi_ctx* ctx, float* out_interior);
+
+/* Inversion::compute_gradient post-processing on the device
+ * (inversion.cu:142-169): crop, gradient/illumination, masked minus z second
+ * difference below zmask [m], scaled by 1/max|g|; *gmax receives max|g|. */
+int fwi_export_gradient(fwi_ctx* ctx, float zmask, float* out_interior, float* gmax);
+
+/* Residual r = obs - cal of the last recorded seismogram, kept on the device
+ * as the backward data, and sum r^2 reduced in FP64 on the device
+ * (Inversion::set_adjoint_source, inversion.cu:190-201; the reference sums
+ * in sequential FP32 on the host). */
+int fwi_residual(fwi_ctx* ctx, const float* obs, double* sum_sq);
+
+/* Autotune tile shape and z-chunking for a pass kind (0 forward RBC,
+ * 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing `steps` steps of
+ * each candidate; cache_path (may be NULL) persists the choice. */
+int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);
+
+/* Current launch configuration: tile rows per block, rows per thread,
+ * z-chunks, shots per block (1 for single-shot contexts). */
+int fwi_get_tuning(fwi_ctx* ctx, int kind, int* by, int* rpt, int* zchunks);
+int fwi_set_tuning(fwi_ctx* ctx, int kind, int by, int rpt, int zchunks);
+
+/* Capture the per-step launches of a full pass in a hipGraph (1) or launch
+ * them on the stream (0, default). */
+int fwi_set_graph(fwi_ctx* ctx, int enable);
+
+/* Device time of the last full pass's step kernels [ms], from HIP events
+ * recorded on the context's stream, and the number of step launches. */
+int fwi_last_pass_timing(fwi_ctx* ctx, double* ms, int* launches);
+
+/* Record a HIP event pair around every step kernel of the following passes
+ * (stream mode only) so that fwi_last_kernel_timing reports the summed
+ * device time of the step kernels alone, without injection kernels. */
+int fwi_set_kernel_timing(fwi_ctx* ctx, int enable);
+int fwi_last_kernel_timing(fwi_ctx* ctx, double* ms, int* launches);
+
+/* Synchronise the context's stream. */
+int fwi_sync(fwi_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FWI3D_H */

@@ -1,0 +1,241 @@
+"""GPU parity of the HIP path against the CPU oracle (run with -m gpu).
+
+The reference arithmetic mode is compared bit for bit: the kernels evaluate
+the same float/double operations in the same order as the oracle (both built
+without FP contraction).  The FP32 mode is held to relative L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.cases import Case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP32 = 1e-5  # north-star FP32 parity bound (relative L2)
+
+
+@pytest.fixture(scope="module")
+def fl():
+    from fwi3d_amd import lib
+    lib.load()
+    return lib
+
+
+def _oracle_forward(case, abc=False, n_end=None):
+    return O.forward(case.vp, case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, case.src, case.rec,
+                     abc_tables=case.abc if abc else None, n_end=n_end)
+
+
+def test_forward_rbc_bit_exact(fl):
+    case = Case()
+    P, Pold, seis = _oracle_forward(case)
+    with case.context(fl.Context) as c:
+        c.forward(fl.BOUNDARY_RBC)
+        g_seis = c.seismogram()
+        g_P = c.get_volume(fl.VOL_FIELD)
+        g_Po = c.get_volume(fl.VOL_FIELD_OLD)
+    assert np.abs(seis).max() > 0
+    assert np.array_equal(g_seis, seis)
+    assert np.array_equal(g_P, P)
+    assert np.array_equal(g_Po, Pold)
+    assert np.all(g_seis[:, 0] == 0)  # sample 0 is never recorded
+
+
+def test_forward_abc_bit_exact(fl):
+    case = Case(amp=0.0, abc_factor=0.05)
+    P, Pold, seis = _oracle_forward(case, abc=True)
+    with case.context(fl.Context, abc=True) as c:
+        c.forward(fl.BOUNDARY_ABC)
+        g_seis = c.seismogram()
+        g_P = c.get_volume(fl.VOL_FIELD)
+        g_Po = c.get_volume(fl.VOL_FIELD_OLD)
+    assert np.array_equal(g_seis, seis)
+    assert np.array_equal(g_P, P)
+    # the device keeps the previous field undamped and damps it on its next
+    # read; damping it here must give the oracle's damped copy
+    d = np.ones_like(g_Po)
+    for k in range(4, case.nyy - 4):
+        for j in range(4, case.nxx - 4):
+            for i in range(4, case.nzz - 4):
+                d[k, j, i] = O.damper_at(case.abc, i, j, k, case.nxx, case.nyy, case.nzz, case.nb)
+    inner = np.zeros_like(g_Po, bool)
+    inner[4:-4, 4:-4, 4:-4] = True
+    assert np.array_equal(np.where(inner, g_Po * d, g_Po), Pold)
+
+
+@pytest.mark.parametrize("fwi", [False, True])
+def test_backward_bit_exact(fl, fwi):
+    case = Case()
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(3)
+    data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+    S, So, R, Ro, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=fwi,
+                                            tlag=case.tlag)
+    with case.context(fl.Context) as c:
+        c.forward(fl.BOUNDARY_RBC)
+        c.set_backward_data(data)
+        (c.grad_backward if fwi else c.rtm_backward)()
+        g_img = c.get_volume(fl.VOL_GRAD if fwi else fl.VOL_IMAGE)
+        g_ill = c.get_volume(fl.VOL_ILLUM)
+        g_S = c.get_volume(fl.VOL_FIELD)
+        g_R = c.get_volume(fl.VOL_RECV)
+        g_Ro = c.get_volume(fl.VOL_RECV_OLD)
+    assert np.abs(image).max() > 0
+    assert np.array_equal(g_ill, illum)
+    assert np.array_equal(g_img, image)
+    assert np.array_equal(g_S, S)
+    assert np.array_equal(g_R, R)
+    assert np.array_equal(g_Ro, Ro)
+
+
+def test_step_ranges_compose(fl):
+    """forward_steps(0,k) + forward_steps(k,N) == forward_steps(0,N) == oracle."""
+    case = Case(nt=60)
+    k = 37
+    P1, Po1, _ = _oracle_forward(case, n_end=k)
+    with case.context(fl.Context) as c:
+        c.forward_steps(fl.BOUNDARY_RBC, 0, k)
+        assert np.array_equal(c.get_volume(fl.VOL_FIELD), P1)
+        assert np.array_equal(c.get_volume(fl.VOL_FIELD_OLD), Po1)
+        c.forward_steps(fl.BOUNDARY_RBC, k, case.steps)
+        P, Pold, _ = _oracle_forward(case)
+        assert np.array_equal(c.get_volume(fl.VOL_FIELD), P)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_all_tilings_identical(fl, kind):
+    """Every tile shape / z-chunking gives the same bits."""
+    case = Case(nx=70, ny=44, nz=40, nt=40, abc_factor=0.05)
+    ref = None
+    P, Pold, seis = _oracle_forward(case, abc=(kind == 1))
+    data = seis.copy()
+    for by, rpt in [(4, 1), (4, 2), (8, 1), (8, 2), (4, 4), (16, 1)]:
+        for zc in (1, 3):
+            with case.context(fl.Context, abc=True) as c:
+                c.set_tuning(kind, by, rpt, zc)
+                if kind < 2:
+                    c.forward(kind)
+                    out = c.seismogram()
+                else:
+                    c.forward(fl.BOUNDARY_RBC)
+                    c.set_backward_data(data)
+                    (c.rtm_backward if kind == 2 else c.grad_backward)()
+                    out = c.get_volume(fl.VOL_IMAGE if kind == 2 else fl.VOL_GRAD)
+            if ref is None:
+                ref = out
+                if kind < 2:
+                    assert np.array_equal(out, seis)
+            assert np.array_equal(out, ref), (by, rpt, zc)
+
+
+def test_fp32_mode_within_tolerance(fl):
+    case = Case(nt=200)
+    P, Pold, seis = _oracle_forward(case)
+    with case.context(fl.Context) as c:
+        c.set_arith(fl.ARITH_FP32)
+        c.forward(fl.BOUNDARY_RBC)
+        g = c.seismogram()
+    err = rel_l2(g, seis)
+    assert err <= TOL_FP32, err
+
+
+def test_graph_matches_stream(fl):
+    case = Case(nt=50)
+    P, Pold, seis = _oracle_forward(case)
+    with case.context(fl.Context) as c:
+        c.set_graph(True)
+        for _ in range(3):  # replays with eager work in between
+            c.forward(fl.BOUNDARY_RBC)
+            assert np.array_equal(c.seismogram(), seis)
+            c.set_backward_data(seis)
+            c.zero_volume(fl.VOL_IMAGE)
+            c.zero_volume(fl.VOL_ILLUM)
+            c.rtm_backward()
+            img_graph = c.get_volume(fl.VOL_IMAGE)
+        c.set_graph(False)
+        c.forward(fl.BOUNDARY_RBC)
+        c.zero_volume(fl.VOL_IMAGE)
+        c.zero_volume(fl.VOL_ILLUM)
+        c.rtm_backward()
+        assert np.array_equal(c.get_volume(fl.VOL_IMAGE), img_graph)
+
+
+def test_model_setup_matches_oracle(fl):
+    case = Case(amp=0.0)
+    with case.context(fl.Context) as c:
+        c.set_model_interior(case.interior)
+        assert np.array_equal(c.get_volume(fl.VOL_MODEL), O.expand_boundary(case.interior, case.nb))
+        c.random_boundary(11, 5, 0.6, float(case.vclamp))
+        want = O.random_boundary(O.expand_boundary(case.interior, case.nb), case.nb, 11, 5, 0.6, case.vclamp)
+        assert np.array_equal(c.get_volume(fl.VOL_MODEL), want)
+
+
+def test_exports_match_oracle(fl):
+    case = Case()
+    rng = np.random.default_rng(5)
+    img = rng.normal(0, 1, (case.nyy, case.nxx, case.nzz)).astype(np.float32)
+    ill = rng.uniform(0.5, 2.0, img.shape).astype(np.float32)
+    with case.context(fl.Context) as c:
+        c.set_volume(fl.VOL_IMAGE, img)
+        c.set_volume(fl.VOL_ILLUM, ill)
+        c.set_volume(fl.VOL_GRAD, img * 3)
+        out = c.export_image()
+        assert np.array_equal(out, O.export_seismic(img, ill, case.nb, case.dh))
+        g, gmax = c.export_gradient(60.0)
+        want, wmax = O.gradient_post(img * 3, ill, case.nb, case.dh, 60.0)
+        assert gmax == wmax
+        assert np.array_equal(g, want)
+
+
+def test_residual_fp64(fl):
+    case = Case(nt=80)
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(9)
+    obs = (seis * 1.1 + rng.normal(0, 1, seis.shape)).astype(np.float32)
+    with case.context(fl.Context) as c:
+        c.forward(fl.BOUNDARY_RBC)
+        s = c.residual(obs)
+    res, want, _ = O.misfit(obs, seis)
+    assert abs(s - want) <= 1e-12 * want
+
+
+def test_errors(fl):
+    case = Case()
+    with case.context(fl.Context) as c:
+        bad = case.vp.copy()
+        bad[3, 3, 3] = np.nan
+        with pytest.raises(fl.FwiError) as e:
+            c.set_model(bad)
+        assert e.value.code == -6
+        with pytest.raises(fl.FwiError) as e:
+            c.set_model(case.vp * 3)
+        assert e.value.code == -3
+        with pytest.raises(fl.FwiError) as e:
+            c.set_source(2, 10, 10)
+        assert e.value.code == -5
+        c.set_model(case.vp)
+        with pytest.raises(fl.FwiError) as e:
+            c.rtm_backward()
+        assert e.value.code == -4
+
+
+@pytest.mark.slow
+def test_two_layer_101_cube_seismogram(fl):
+    """The 101^3 two-layer workload (nb = 50, 1000 steps, 81x81 receivers):
+    full-size seismogram, bit for bit against the oracle."""
+    from fwi3d_amd import configs
+    w = configs.two_layer_101()
+    vp = O.random_boundary(O.expand_boundary(w.interior, w.nb), w.nb, w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+    P, Pold, seis = O.forward(vp, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src, w.rec)
+    with fl.Context(w.nxx, w.nyy, w.nzz, w.nb, w.dh, w.dt, w.nt, w.tlag) as c:
+        c.set_model_interior(w.interior)
+        c.random_boundary(w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+        assert np.array_equal(c.get_volume(fl.VOL_MODEL), vp)
+        c.set_wavelet(w.wavelet)
+        c.set_source(*w.src)
+        c.set_receivers(w.rec)
+        c.forward(fl.BOUNDARY_RBC)
+        g = c.seismogram()
+    assert g.shape == (81 * 81, 1000)
+    assert np.array_equal(g, seis), rel_l2(g, seis)
