@@ -101,6 +101,7 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
     constexpr int TY = BY * RPT;
     constexpr int LW = 64 + 8;
     constexpr int LH = TY + 8;
+    constexpr int HY = (8 + BY - 1) / BY;  // y-halo rows loaded per wave
     __shared__ float lds[2][SB * NF][LH][LW];
 
     const Layout& L = a.L;
@@ -148,6 +149,11 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
         if (zb >= ze) continue;  // uniform per block
         const bool xin = x >= 4 && x < L.nx - 4;
         const bool xld = x < L.nx;
+        // x-halo lanes: 0..3 load x-4 (left), 60..63 load x+4 (right)
+        const int hxx = tx < 4 ? x - 4 : x + 4;
+        const int hxc = tx < 4 ? tx : tx + 8;
+        const bool hx_lane = tx < 4 || tx >= 60;
+        const bool hxok = hx_lane && hxx >= 0 && hxx < L.nx;
 
         int yr[RPT];
         bool act[RPT], yld[RPT];
@@ -156,6 +162,15 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             yr[r] = y0 + ty + r * BY;
             act[r] = xin && yr[r] < L.ny - 4;
             yld[r] = xld && yr[r] < L.ny;
+        }
+        int hy_y[HY], hy_row[HY];
+        bool hy_ok[HY];
+#pragma unroll
+        for (int m = 0; m < HY; ++m) {
+            const int h = ty + m * BY;
+            hy_y[m] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+            hy_row[m] = h < 4 ? h : TY + h;
+            hy_ok[m] = h < 8 && xld && hy_y[m] < L.ny;
         }
 
         const float* fld[SB][NF];
@@ -171,8 +186,62 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
         }
         const long gbase = (long)grp * a.istride;
 
-        // prime the register queues with planes zb-4 .. zb+3
+        // Pipeline registers: q = z-samples z-4..z+4 of the stencil fields;
+        // hx/hy = halo values of the next plane; o/v/ro/im/il = per-cell
+        // streams of the plane being computed, *_n = of the next plane.
         float q[SB][NF][RPT][9];
+        float qn[SB][NF][RPT];
+        float hx[SB][NF][RPT];
+        float hy[SB][NF][HY];
+        float o[SB][RPT], v[SB][RPT], ro[SB][RPT], im[RPT], il[RPT];
+        float o_n[SB][RPT], v_n[SB][RPT], ro_n[SB][RPT], im_n[RPT], il_n[RPT];
+
+        auto load_plane = [&](int zz, float (&oo)[SB][RPT], float (&vv)[SB][RPT], float (&rr)[SB][RPT],
+                              float (&ii)[RPT], float (&ll)[RPT]) {
+            // halos of plane zz and the streamed per-cell values of plane zz
+#pragma unroll
+            for (int s = 0; s < SB; ++s)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r)
+                        hx[s][f][r] = (hxok && yr[r] < L.ny) ? fld[s][f][L.at(hxx, yr[r], zz)] : 0.0f;
+#pragma unroll
+                    for (int m = 0; m < HY; ++m) hy[s][f][m] = hy_ok[m] ? fld[s][f][L.at(x, hy_y[m], zz)] : 0.0f;
+                }
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const long off = L.at(x, yr[r], zz);
+#pragma unroll
+                for (int s = 0; s < SB; ++s) {
+                    oo[s][r] = act[r] ? __builtin_nontemporal_load(a.old + sbase[s] + off) : 0.0f;
+                    vv[s][r] = act[r] ? __builtin_nontemporal_load(a.vp + sbase[s] + off) : 0.0f;
+                    if constexpr (BWD) rr[s][r] = act[r] ? __builtin_nontemporal_load(a.rold + sbase[s] + off) : 0.0f;
+                }
+                if constexpr (BWD) {
+                    ii[r] = act[r] ? __builtin_nontemporal_load(a.image + gbase + off) : 0.0f;
+                    ll[r] = act[r] ? __builtin_nontemporal_load(a.illum + gbase + off) : 0.0f;
+                }
+            }
+        };
+        auto stage_plane = [&](int buf, int qi) {
+            // centre values (queue slot qi) and the halos into LDS buffer buf
+#pragma unroll
+            for (int s = 0; s < SB; ++s)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r) {
+                        lds[buf][s * NF + f][ty + r * BY + 4][tx + 4] = q[s][f][r][qi];
+                        if (hx_lane) lds[buf][s * NF + f][ty + r * BY + 4][hxc] = hx[s][f][r];
+                    }
+#pragma unroll
+                    for (int m = 0; m < HY; ++m)
+                        if (ty + m * BY < 8) lds[buf][s * NF + f][hy_row[m]][tx + 4] = hy[s][f][m];
+                }
+        };
+
+        // prologue: planes zb-4 .. zb+4 into the queue, plane zb into LDS
 #pragma unroll
         for (int s = 0; s < SB; ++s)
 #pragma unroll
@@ -180,66 +249,35 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
 #pragma unroll
                 for (int r = 0; r < RPT; ++r)
 #pragma unroll
-                    for (int d = 0; d < 8; ++d)
+                    for (int d = 0; d < 9; ++d)
                         q[s][f][r][d] = yld[r] ? fld[s][f][L.at(x, yr[r], zb - 4 + d)] : 0.0f;
+        load_plane(zb, o, v, ro, im, il);
+        stage_plane(0, 4);
+        __syncthreads();
 
         for (int z = zb; z < ze; ++z) {
             const int buf = (z - zb) & 1;
-            // plane z+4 into the queue, plane z halos into LDS
-#pragma unroll
-            for (int s = 0; s < SB; ++s)
-#pragma unroll
-                for (int f = 0; f < NF; ++f)
-#pragma unroll
-                    for (int r = 0; r < RPT; ++r) {
-                        q[s][f][r][8] = yld[r] ? fld[s][f][L.at(x, yr[r], z + 4)] : 0.0f;
-                        lds[buf][s * NF + f][ty + r * BY + 4][tx + 4] = q[s][f][r][4];
-                    }
-            if (tx < 4) {
-                const int xx = x - 4;
+            const bool more = z + 1 < ze;  // uniform per block
+            // 1. prefetch plane z+1 (queue head z+5, halos, streams)
+            if (more) {
 #pragma unroll
                 for (int s = 0; s < SB; ++s)
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
 #pragma unroll
                         for (int r = 0; r < RPT; ++r)
-                            lds[buf][s * NF + f][ty + r * BY + 4][tx] =
-                                (xx >= 0 && yr[r] < L.ny) ? fld[s][f][L.at(xx, yr[r], z)] : 0.0f;
-            } else if (tx >= 60) {
-                const int xx = x + 4;
-#pragma unroll
-                for (int s = 0; s < SB; ++s)
-#pragma unroll
-                    for (int f = 0; f < NF; ++f)
-#pragma unroll
-                        for (int r = 0; r < RPT; ++r)
-                            lds[buf][s * NF + f][ty + r * BY + 4][tx + 8] =
-                                (xx < L.nx && yr[r] < L.ny) ? fld[s][f][L.at(xx, yr[r], z)] : 0.0f;
+                            qn[s][f][r] = yld[r] ? fld[s][f][L.at(x, yr[r], z + 5)] : 0.0f;
+                load_plane(z + 1, o_n, v_n, ro_n, im_n, il_n);
             }
-#pragma unroll
-            for (int h = ty; h < 8; h += BY) {
-                const int yy = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
-                const int lr = h < 4 ? h : TY + h;
-                const bool ok = xld && yy < L.ny;
-#pragma unroll
-                for (int s = 0; s < SB; ++s)
-#pragma unroll
-                    for (int f = 0; f < NF; ++f)
-                        lds[buf][s * NF + f][lr][tx + 4] = ok ? fld[s][f][L.at(x, yy, z)] : 0.0f;
-            }
-            __syncthreads();
 
+            // 2. compute plane z
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 if (!act[r]) continue;
                 const long off = L.at(x, yr[r], z);
                 const int lr = ty + r * BY + 4;
                 const int lc = tx + 4;
-                float il = 0.0f, im = 0.0f;
-                if constexpr (BWD) {
-                    il = a.illum[gbase + off];
-                    im = a.image[gbase + off];
-                }
+                float ill = il[r], img = im[r];
 #pragma unroll
                 for (int s = 0; s < SB; ++s) {
                     const float(*P)[LW] = lds[buf][s * NF];
@@ -250,22 +288,22 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                                                 P[lr - 3][lc], P[lr + 3][lc], P[lr - 4][lc], P[lr + 4][lc], a.idh2);
                     const float* qz = q[s][0][r];
                     const float dzz = d2<ARITH>(c, qz[3], qz[5], qz[2], qz[6], qz[1], qz[7], qz[0], qz[8], a.idh2);
-                    const float v = a.vp[sbase[s] + off];
-                    const float vp2 = v * v;
+                    const float vel = v[s][r];
+                    const float vp2 = vel * vel;
                     const float K = a.dt2 * vp2;
                     if constexpr (!BWD) {
-                        float o = a.old[sbase[s] + off];
+                        float oo = o[s][r];
                         float dmp = 1.0f;
                         if constexpr (ABC) {
                             dmp = damper(a.b1d, a.nb, x, yr[r], z, L);
-                            o = o * dmp;
+                            oo = oo * dmp;
                         }
                         float nv = K * ((dxx + dyy) + dzz);
                         nv = nv + 2.0f * c;
-                        nv = nv - o;
+                        nv = nv - oo;
                         if constexpr (ABC) nv = nv * dmp;
                         if (off == src_off[s]) nv = nv + a.inj;
-                        a.old[sbase[s] + off] = nv;
+                        __builtin_nontemporal_store(nv, a.old + sbase[s] + off);
                     } else {
                         const float(*R)[LW] = lds[buf][s * NF + 1];
                         const float cr = q[s][1][r][4];
@@ -280,39 +318,59 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                             d2<ARITH>(cr, rz[3], rz[5], rz[2], rz[6], rz[1], rz[7], rz[0], rz[8], a.idh2);
                         float sn = K * ((dxx + dyy) + dzz);
                         sn = sn + 2.0f * c;
-                        sn = sn - a.old[sbase[s] + off];
+                        sn = sn - o[s][r];
                         float rn = K * ((rxx + ryy) + rzz);
                         rn = rn + 2.0f * cr;
-                        rn = rn - a.rold[sbase[s] + off];
-                        a.old[sbase[s] + off] = sn;
-                        a.rold[sbase[s] + off] = rn;
-                        il = il + sn * sn;
+                        rn = rn - ro[s][r];
+                        __builtin_nontemporal_store(sn, a.old + sbase[s] + off);
+                        __builtin_nontemporal_store(rn, a.rold + sbase[s] + off);
+                        ill = ill + sn * sn;
                         if constexpr (MODE == BWD_RTM) {
-                            im = im + sn * cr;
+                            img = img + sn * cr;
                         } else {
                             float gterm = a.dt * cr;
                             gterm = gterm * (dxx + dzz);
                             gterm = gterm * vp2;
-                            im = im + gterm;
+                            img = img + gterm;
                         }
                     }
                 }
                 if constexpr (BWD) {
-                    a.illum[gbase + off] = il;
-                    a.image[gbase + off] = im;
+                    __builtin_nontemporal_store(ill, a.illum + gbase + off);
+                    __builtin_nontemporal_store(img, a.image + gbase + off);
                 }
             }
 
+            // 3. stage plane z+1 into the other LDS buffer, advance the queue
+            if (more) {
+                stage_plane(buf ^ 1, 5);
 #pragma unroll
-            for (int s = 0; s < SB; ++s)
+                for (int s = 0; s < SB; ++s) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f)
+                    for (int f = 0; f < NF; ++f)
 #pragma unroll
-                    for (int r = 0; r < RPT; ++r)
+                        for (int r = 0; r < RPT; ++r) {
 #pragma unroll
-                        for (int d = 0; d < 8; ++d) q[s][f][r][d] = q[s][f][r][d + 1];
+                            for (int d = 0; d < 8; ++d) q[s][f][r][d] = q[s][f][r][d + 1];
+                            q[s][f][r][8] = qn[s][f][r];
+                        }
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r) {
+                        o[s][r] = o_n[s][r];
+                        v[s][r] = v_n[s][r];
+                        if constexpr (BWD) ro[s][r] = ro_n[s][r];
+                    }
+                }
+                if constexpr (BWD) {
+#pragma unroll
+                    for (int r = 0; r < RPT; ++r) {
+                        im[r] = im_n[r];
+                        il[r] = il_n[r];
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();  // LDS reuse by the next item
     }
 }
 
