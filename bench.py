@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--pass-steps", type=int, default=0,
+                    help="profiling only: cut each pass to this many time steps")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="profiling only: also run a plain 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -107,11 +111,32 @@ def main():
     log("data ready")
 
     def step():
-        ctx.forward(lib.BOUNDARY_RBC)
+        if args.pass_steps:
+            ctx.forward_steps(lib.BOUNDARY_RBC, 0, args.pass_steps)
+        else:
+            ctx.forward(lib.BOUNDARY_RBC)
         f = ctx.last_pass_timing() + ctx.last_kernel_timing()
-        ctx.rtm_backward()
+        if args.pass_steps:
+            ctx.backward_steps(0, 0, args.pass_steps)
+        else:
+            ctx.rtm_backward()
         b = ctx.last_pass_timing() + ctx.last_kernel_timing()
         return f, b
+
+    if args.copy_probe:
+        import torch
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / 10 * 1e3
+        log("copy probe: 1 GiB copy %.3f ms -> %.1f GB/s (read+write)" % (ms, 2 * (1 << 30) / (ms * 1e-3) / 1e9))
+        del src, dst
 
     for i in range(args.warmup):
         step()
@@ -135,7 +160,7 @@ def main():
     wall = time.perf_counter() - t0
 
     cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
-    N = w.steps
+    N = args.pass_steps or w.steps
     updates_per_step = cells * N * 3
     sec_per_step = wall / args.steps
     value = updates_per_step / sec_per_step / 1e9

@@ -123,18 +123,22 @@ class Workload:
         return (self.nxx - 8) * (self.nyy - 8) * (self.nzz - 8) * self.steps * fields
 
     # -- reference file layout ---------------------------------------------
-    def write(self, folder, prefix=None):
+    def write(self, folder, prefix=None, model=None, kind="true", max_iteration=5, batch_shots=1, depth_mask=40.0,
+              abc_factor=0.004):
         """Write model, SPS/RPS and parameters.txt under `folder`; returns the
-        parameters.txt path.  Receiver patches that differ per shot are not
-        expressible in the reference's single RPS file: the union is written."""
+        parameters.txt path.  `model` replaces the interior written as the
+        model file (e.g. a smoothed starting model for inversion, named with
+        `kind`).  The reference reads one RPS file for all shots, so shot 0's
+        receivers are written."""
         prefix = prefix or self.name
         os.makedirs(folder, exist_ok=True)
         for sub in ("inputs/models", "inputs/geometry", "inputs/data", "outputs/models", "outputs/residuo",
                     "outputs/seismic"):
             os.makedirs(os.path.join(folder, sub), exist_ok=True)
-        model = os.path.join(folder, "inputs/models",
-                             "%s_true_%dx%dx%d_%dm.bin" % (prefix, self.nz, self.nx, self.ny, int(self.dh)))
-        self.interior.astype(np.float32).tofile(model)
+        model_path = os.path.join(folder, "inputs/models",
+                                  "%s_%s_%dx%dx%d_%dm.bin" % (prefix, kind, self.nz, self.nx, self.ny, int(self.dh)))
+        (self.interior if model is None else np.asarray(model)).astype(np.float32).tofile(model_path)
+        model = model_path
         sps = os.path.join(folder, "inputs/geometry", prefix + "_SPS.txt")
         rps = os.path.join(folder, "inputs/geometry", prefix + "_RPS.txt")
         np.savetxt(sps, self.shots, fmt="%.2f", delimiter=",")
@@ -145,7 +149,7 @@ class Workload:
         text = PARAMS_TEMPLATE.format(
             nx=self.nx, ny=self.ny, nz=self.nz, dh=self.dh, model=model, sps=sps, rps=rps, nt=self.nt,
             dt=self.dt, fmax=self.fmax, length=length, folder=folder, nrec=nrec, dtus=dtus, seed=self.rbc_seed,
-            amp=self.rbc_amp)
+            amp=self.rbc_amp, maxit=max_iteration, batch=batch_shots, zmask=depth_mask, factor=abc_factor)
         path = os.path.join(folder, "parameters.txt")
         with open(path, "w") as f:
             f.write(text)
@@ -181,7 +185,7 @@ time_spacing = {dt}                 # [s]
 max_frequency = {fmax}              # [Hz]
 
 abc_length = {length}               # [m]
-abc_factor = 0.004
+abc_factor = {factor}
 
 mod_output_folder = {folder}/inputs/data/
 
@@ -189,9 +193,9 @@ mod_output_folder = {folder}/inputs/data/
 # Inversion parameters --------------------------------------
 #------------------------------------------------------------
 
-max_iteration = 5
+max_iteration = {maxit}
 
-depth_mask = 40                     # [m]
+depth_mask = {zmask}                 # [m]
 
 inv_rbc_ratio = 100                 # [m]
 inv_rbc_varVp = 1000                # [m/s]
@@ -221,6 +225,7 @@ mig_output_folder = {folder}/outputs/seismic/
 
 rbc_seed = {seed}
 rbc_amplitude = {amp}
+batch_shots = {batch}
 """
 
 
@@ -282,6 +287,21 @@ def gradient_401_16(interior=None):
             X, Y = np.meshgrid(gx, gy, indexing="ij")
             recs.append(np.stack([X.ravel(), Y.ravel(), np.full(X.size, 20.0)], axis=1))
     return Workload("gradient16", interior, 50, 10.0, 0.8e-3, 3000, 60.0, np.array(shots), recs)
+
+
+def tiny(nshots=2, nx=40, ny=36, nz=30, nt=160):
+    """A small two-layer model for driver tests: nb 8, dt 1 ms, 45 Hz."""
+    dh = 10.0
+    z = np.arange(nz) * dh
+    col = np.where(z < nz * dh / 2, 1600.0, 2200.0).astype(np.float32)
+    interior = np.ascontiguousarray(np.broadcast_to(col, (ny, nx, nz)))
+    xs = np.linspace(80.0, (nx - 9) * dh, nshots)
+    shots = np.stack([xs, np.full(nshots, (ny // 2) * dh), np.full(nshots, 20.0)], axis=1)
+    gx = np.arange(20.0, (nx - 2) * dh, 30.0)
+    gy = np.arange(20.0, (ny - 2) * dh, 30.0)
+    X, Y = np.meshgrid(gx, gy, indexing="ij")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 20.0)], axis=1)
+    return Workload("tiny", interior, 8, dh, 1e-3, nt, 45.0, shots, [rec] * nshots)
 
 
 WORKLOADS = {"two_layer_101": two_layer_101, "gradient_401": gradient_401, "gradient_401_16": gradient_401_16}

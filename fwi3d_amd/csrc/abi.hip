@@ -1,18 +1,21 @@
 // abi.hip -- the C ABI of libfwi3d_hip.so (declared in include/fwi3d.h).
 //
-// A context owns one HIP stream and every device buffer of a shot: padded
-// model, source-side fields, receiver-side fields, image / illumination /
-// gradient accumulators, the time-major seismogram and the backward data.
-// The time loops of the reference's forward_solver (modeling.cu:309-322)
-// and backward_propagation (migration.cu:61-81, inversion.cu:235-253) are
-// issued here, one step kernel per time step (optionally captured in a
-// hipGraph).
+// A context owns one HIP stream and the device buffers of `nshots` shot
+// slots: per-slot padded model (each shot has its own random boundary),
+// source-side and receiver-side fields, the flattened receiver lists of all
+// slots, the time-major seismogram and backward data, and the image /
+// illumination / gradient accumulators (one partial per group of shots that
+// share a block; folded into group 0 after a pass).  The time loops of the
+// reference's forward_solver (modeling.cu:309-322) and backward_propagation
+// (migration.cu:61-81, inversion.cu:235-253) are issued here: one step
+// kernel per time step advances every slot, optionally captured in a
+// hipGraph.  A single-shot context is the nshots = 1 case; the entry points
+// without a slot argument address slot 0.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
-#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -29,39 +32,41 @@ struct fwi_ctx {
     hipStream_t st = nullptr;
     fwi_grid g{};
     Layout L{};
+    int nshots = 1;
     int arith = FWI_ARITH_REFERENCE;
     float idh2 = 0, idh3 = 0, dt2 = 0;
 
-    float* vp = nullptr;
-    float* f[2] = {nullptr, nullptr};
-    int fcur = 0;  // f[fcur] = d_P (current), f[1-fcur] = d_Pold
-    float* r[2] = {nullptr, nullptr};
-    int rcur = 0;  // r[rcur] = d_Pr, r[1-rcur] = d_Prold
-    float* image = nullptr;
+    float* vp = nullptr;                 // nshots volumes
+    float* f[2] = {nullptr, nullptr};    // f[fcur] = d_P (current), f[1-fcur] = d_Pold
+    int fcur = 0;
+    float* r[2] = {nullptr, nullptr};    // r[rcur] = d_Pr, r[1-rcur] = d_Prold
+    int rcur = 0;
+    float* image = nullptr;              // nshots partial volumes each; group 0 = result
     float* illum = nullptr;
     float* grad = nullptr;
-    float* staging = nullptr;  // one padded volume in the reference layout
+    float* staging = nullptr;            // one padded volume, reference layout
 
     std::vector<float> wavelet;
     float* b1d = nullptr;
     std::vector<float> b1d_h;
-    bool have_model = false, have_src = false;
+    bool have_model = false;
 
-    int sx = -1, sy = -1, sz = -1;
-    long* src_off = nullptr;  // [1]
-    int nrec = 0;
-    std::vector<int> rx, ry, rz;
-    long* rec_off = nullptr;
+    std::vector<int> sx, sy, sz;         // per slot, -1 = unset
+    long* src_off = nullptr;             // device [nshots], offsets within a slot
+    std::vector<std::vector<int>> rx, ry, rz;
+    bool geom_dirty = true;
+    std::vector<int> rec_start_h;        // nshots + 1
+    int nrec_total = 0;
+    long* rec_off = nullptr;             // absolute offsets (slot*vol + cell)
     float* rec_damp = nullptr;
-    int* rec_start = nullptr;  // {0, nrec}
+    int* rec_start = nullptr;
     int ncells = 0;
     long* cell_off = nullptr;
     int* cell_start = nullptr;
     int* cell_rec = nullptr;
-
-    float* seis_tm = nullptr;  // [t][r]
-    float* data_tm = nullptr;  // [t][r] backward data
-    float* seis_tmp = nullptr; // transposes
+    float* seis_tm = nullptr;            // [t][q] over all slots' receivers
+    float* data_tm = nullptr;            // [t][q] backward data
+    float* seis_tmp = nullptr;           // 2 * nt * nrec_total scratch for transposes
     double* red_partial = nullptr;
     double* red_result = nullptr;
 
@@ -70,9 +75,9 @@ struct fwi_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
-    int fwd_done = 0;  // 1 after a full forward pass
-    bool ktime = false;             // per-step-kernel events
-    std::vector<hipEvent_t> kev;    // 2 per step of the last pass
+    int fwd_done = 0;                    // 1 after a full forward pass
+    bool ktime = false;                  // per-step-kernel events
+    std::vector<hipEvent_t> kev;
     int kev_used = 0;
     double kernel_ms = 0.0;
     std::string err;
@@ -95,76 +100,148 @@ int fail(fwi_ctx* c, int code, const std::string& msg) {
             return fail(ctx, FWI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));             \
     } while (0)
 
-long nvol_host(const fwi_grid& g) { return (long)g.nxx * g.nyy * g.nzz; }
+#define CHKRC(expr)             \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_) return rc_;    \
+    } while (0)
 
-int alloc_vol(fwi_ctx* c, float** p) {
+long nvol_host(const fwi_grid& g) { return (long)g.nxx * g.nyy * g.nzz; }
+long ninterior(const fwi_grid& g) {
+    return (long)(g.nxx - 2 * g.nb) * (g.nyy - 2 * g.nb) * (g.nzz - 2 * g.nb);
+}
+
+int alloc_zero(fwi_ctx* c, float** p, long nfloats) {
     if (*p) return FWI_OK;
-    HIPCHK(c, hipMalloc((void**)p, c->L.vol * sizeof(float)));
-    HIPCHK(c, hipMemsetAsync(*p, 0, c->L.vol * sizeof(float), c->st));
+    HIPCHK(c, hipMalloc((void**)p, nfloats * sizeof(float)));
+    HIPCHK(c, hipMemsetAsync(*p, 0, nfloats * sizeof(float), c->st));
     return FWI_OK;
 }
+
+int alloc_slots(fwi_ctx* c, float** p) { return alloc_zero(c, p, c->L.vol * c->nshots); }
 
 // Host twin of the device damper (same float operations).
 float host_damper(const std::vector<float>& b1, int nb, int x, int y, int z, const Layout& L) {
-    const int di = z < nb ? z : (z >= L.nz - nb ? L.nz - 1 - z : -1);
-    const int dj = x < nb ? x : (x >= L.nx - nb ? L.nx - 1 - x : -1);
-    const int dk = y < nb ? y : (y >= L.ny - nb ? L.ny - 1 - y : -1);
-    const int cnt = (di >= 0) + (dj >= 0) + (dk >= 0);
+    auto axis = [&](int q, int n, float* b) {
+        const int d = q < nb ? q : (q >= n - nb ? n - 1 - q : -1);
+        *b = d >= 0 ? b1[d] : 0.0f;
+        return d >= 0;
+    };
+    float bi, bj, bk;
+    const bool oi = axis(z, L.nz, &bi), oj = axis(x, L.nx, &bj), ok = axis(y, L.ny, &bk);
+    const int cnt = (int)oi + (int)oj + (int)ok;
     if (cnt == 0) return 1.0f;
-    if (cnt == 1) return b1[di >= 0 ? di : (dj >= 0 ? dj : dk)];
+    if (cnt == 1) return oi ? bi : (oj ? bj : bk);
     if (cnt == 2) {
-        const int a = di >= 0 ? di : dj;
-        const int b = dk >= 0 ? dk : dj;
-        return (b1[a] + b1[b]) - 1.0f;
+        const float a = oi ? bi : bj;
+        const float b = ok ? bk : bj;
+        return (a + b) - 1.0f;
     }
-    const float t1 = b1[di] + b1[dj], t2 = b1[dj] + b1[dk], t3 = b1[di] + b1[dk];
+    const float t1 = bi + bj, t2 = bj + bk, t3 = bi + bk;
     return ((t1 + t2) + t3) - 5.0f;
-}
-
-int refresh_rec_damp(fwi_ctx* c) {
-    if (c->nrec == 0 || c->b1d_h.empty()) return FWI_OK;
-    std::vector<float> d(c->nrec);
-    for (int q = 0; q < c->nrec; ++q) d[q] = host_damper(c->b1d_h, c->g.nb, c->rx[q], c->ry[q], c->rz[q], c->L);
-    if (!c->rec_damp) HIPCHK(c, hipMalloc((void**)&c->rec_damp, c->nrec * sizeof(float)));
-    HIPCHK(c, hipMemcpy(c->rec_damp, d.data(), c->nrec * sizeof(float), hipMemcpyHostToDevice));
-    return FWI_OK;
 }
 
 bool in_updated(const fwi_ctx* c, int x, int y, int z) {
     return x >= 4 && x < c->g.nxx - 4 && y >= 4 && y < c->g.nyy - 4 && z >= 4 && z < c->g.nzz - 4;
 }
 
-float* vol_ptr(fwi_ctx* c, int which, bool alloc, int* rc) {
+int check_slot(fwi_ctx* c, int slot) {
+    if (slot < 0 || slot >= c->nshots) return fail(c, FWI_ERR_ARG, "shot slot out of range");
+    return FWI_OK;
+}
+
+// (Re)build the flattened receiver arrays of all slots after a geometry
+// change; recorded seismograms and uploaded data are cleared.
+int sync_geometry(fwi_ctx* c) {
+    if (!c->geom_dirty) return FWI_OK;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (void* p : {(void*)c->rec_off, (void*)c->rec_damp, (void*)c->seis_tm, (void*)c->data_tm, (void*)c->seis_tmp,
+                    (void*)c->cell_off, (void*)c->cell_start, (void*)c->cell_rec})
+        if (p) HIPCHK(c, hipFree(p));
+    c->rec_off = nullptr;
+    c->rec_damp = nullptr;
+    c->seis_tm = c->data_tm = c->seis_tmp = nullptr;
+    c->cell_off = nullptr;
+    c->cell_start = c->cell_rec = nullptr;
+    c->ncells = 0;
+    const int S = c->nshots;
+    c->rec_start_h.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) c->rec_start_h[s + 1] = c->rec_start_h[s] + (int)c->rx[s].size();
+    const int n = c->nrec_total = c->rec_start_h[S];
+    HIPCHK(c, hipMemcpy(c->rec_start, c->rec_start_h.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+    c->geom_dirty = false;
+    if (n == 0) return FWI_OK;
+    std::vector<long> off(n);
+    std::vector<float> damp(n, 1.0f);
+    for (int s = 0; s < S; ++s)
+        for (size_t q = 0; q < c->rx[s].size(); ++q) {
+            const int k = c->rec_start_h[s] + (int)q;
+            off[k] = (long)s * c->L.vol + c->L.at(c->rx[s][q], c->ry[s][q], c->rz[s][q]);
+            if (!c->b1d_h.empty()) damp[k] = host_damper(c->b1d_h, c->g.nb, c->rx[s][q], c->ry[s][q], c->rz[s][q], c->L);
+        }
+    // receivers grouped by cell, ascending receiver order inside a cell
+    std::vector<int> order(n);
+    for (int q = 0; q < n; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return off[a] < off[b]; });
+    std::vector<long> coff;
+    std::vector<int> cstart;
+    for (int q = 0; q < n; ++q)
+        if (q == 0 || off[order[q]] != off[order[q - 1]]) {
+            coff.push_back(off[order[q]]);
+            cstart.push_back(q);
+        }
+    cstart.push_back(n);
+    c->ncells = (int)coff.size();
+    const long ns = (long)c->g.nt * n;
+    HIPCHK(c, hipMalloc((void**)&c->rec_off, n * sizeof(long)));
+    HIPCHK(c, hipMalloc((void**)&c->rec_damp, n * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&c->cell_off, coff.size() * sizeof(long)));
+    HIPCHK(c, hipMalloc((void**)&c->cell_start, cstart.size() * sizeof(int)));
+    HIPCHK(c, hipMalloc((void**)&c->cell_rec, n * sizeof(int)));
+    HIPCHK(c, hipMalloc((void**)&c->seis_tm, ns * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&c->data_tm, ns * sizeof(float)));
+    HIPCHK(c, hipMalloc((void**)&c->seis_tmp, 2 * ns * sizeof(float)));
+    HIPCHK(c, hipMemcpy(c->rec_off, off.data(), n * sizeof(long), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->rec_damp, damp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->cell_off, coff.data(), coff.size() * sizeof(long), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->cell_start, cstart.data(), cstart.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->cell_rec, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->seis_tm, 0, ns * sizeof(float)));
+    HIPCHK(c, hipMemset(c->data_tm, 0, ns * sizeof(float)));
+    return FWI_OK;
+}
+
+float* vol_ptr(fwi_ctx* c, int slot, int which, bool alloc, int* rc) {
     *rc = FWI_OK;
     float** p = nullptr;
+    bool per_slot = true;
     switch (which) {
         case FWI_VOL_MODEL: p = &c->vp; break;
         case FWI_VOL_FIELD: p = &c->f[c->fcur]; break;
         case FWI_VOL_FIELD_OLD: p = &c->f[1 - c->fcur]; break;
         case FWI_VOL_RECV: p = &c->r[c->rcur]; break;
         case FWI_VOL_RECV_OLD: p = &c->r[1 - c->rcur]; break;
-        case FWI_VOL_IMAGE: p = &c->image; break;
-        case FWI_VOL_ILLUM: p = &c->illum; break;
-        case FWI_VOL_GRAD: p = &c->grad; break;
+        case FWI_VOL_IMAGE: p = &c->image; per_slot = false; break;
+        case FWI_VOL_ILLUM: p = &c->illum; per_slot = false; break;
+        case FWI_VOL_GRAD: p = &c->grad; per_slot = false; break;
         default: *rc = fail(c, FWI_ERR_ARG, "unknown volume id"); return nullptr;
     }
     if (!*p && alloc) {
-        *rc = alloc_vol(c, p);
+        *rc = alloc_slots(c, p);
         if (*rc) return nullptr;
     }
-    return *p;
+    if (!*p) return nullptr;
+    return *p + (per_slot ? (long)slot * c->L.vol : 0);
 }
 
 int ensure_backward_buffers(fwi_ctx* c, int kind) {
-    int rc;
-    if ((rc = alloc_vol(c, &c->r[0]))) return rc;
-    if ((rc = alloc_vol(c, &c->r[1]))) return rc;
-    if ((rc = alloc_vol(c, &c->illum))) return rc;
-    if ((rc = alloc_vol(c, kind == 0 ? &c->image : &c->grad))) return rc;
+    CHKRC(alloc_slots(c, &c->r[0]));
+    CHKRC(alloc_slots(c, &c->r[1]));
+    CHKRC(alloc_slots(c, &c->illum));
+    CHKRC(alloc_slots(c, kind == 0 ? &c->image : &c->grad));
     return FWI_OK;
 }
 
-// Event pair slot for step `i` of the current pass (kernel timing mode).
 int kernel_event(fwi_ctx* c, int slot, hipEvent_t* ev) {
     while ((int)c->kev.size() <= slot) {
         hipEvent_t e;
@@ -177,14 +254,14 @@ int kernel_event(fwi_ctx* c, int slot, hipEvent_t* ev) {
 
 int launch_step_timed(fwi_ctx* c, int mode, const Tuning& t, const StepArgs& a) {
     if (!c->ktime || c->use_graph) {
-        HIPCHK(c, fwi::launch_step(mode, c->arith, t, 1, a, c->st));
+        HIPCHK(c, fwi::launch_step(mode, c->arith, t, a, c->st));
         return FWI_OK;
     }
     hipEvent_t e0, e1;
-    int rc;
-    if ((rc = kernel_event(c, 2 * c->kev_used, &e0)) || (rc = kernel_event(c, 2 * c->kev_used + 1, &e1))) return rc;
+    CHKRC(kernel_event(c, 2 * c->kev_used, &e0));
+    CHKRC(kernel_event(c, 2 * c->kev_used + 1, &e1));
     HIPCHK(c, hipEventRecord(e0, c->st));
-    HIPCHK(c, fwi::launch_step(mode, c->arith, t, 1, a, c->st));
+    HIPCHK(c, fwi::launch_step(mode, c->arith, t, a, c->st));
     HIPCHK(c, hipEventRecord(e1, c->st));
     c->kev_used++;
     return FWI_OK;
@@ -194,7 +271,7 @@ StepArgs base_args(fwi_ctx* c) {
     StepArgs a{};
     a.L = c->L;
     a.sstride = c->L.vol;
-    a.nshots = 1;
+    a.nshots = c->nshots;
     a.vp = c->vp;
     a.istride = c->L.vol;
     a.idh2 = c->idh2;
@@ -208,31 +285,33 @@ StepArgs base_args(fwi_ctx* c) {
     return a;
 }
 
-// Forward steps [n0, n1) from the current state (see fwi_forward_steps).
-// record = false skips the receiver gather (autotuning trial steps).
+// Forward steps [n0, n1) of every slot from the current state: the current
+// field excludes the source term of step n0, which is added first; each step
+// kernel adds the source term of the next step to the field it writes
+// (except after the last step).  record = false skips the receiver gather.
 int run_forward(fwi_ctx* c, int boundary, int n0, int n1, bool record = true) {
     const int mode = boundary == FWI_BOUNDARY_ABC ? fwi::FWD_ABC : fwi::FWD_RBC;
     const Tuning& t = c->tune[mode];
     StepArgs a = base_args(c);
     if (boundary == FWI_BOUNDARY_ABC) a.rec_damp = c->rec_damp;
-    const long src = (long)c->sx + (long)c->sy * c->L.px + (long)c->sz * c->L.pp;
     if (n0 < n1 && n0 < c->g.nt)
-        HIPCHK(c, fwi::launch_add_point(c->f[c->fcur], src, c->idh3 * c->wavelet[n0], c->st));
+        HIPCHK(c, fwi::launch_add_points(c->f[c->fcur], c->src_off, c->nshots, c->L.vol, c->idh3 * c->wavelet[n0],
+                                         c->st));
     for (int n = n0; n < n1; ++n) {
         a.cur = c->f[c->fcur];
         a.old = c->f[1 - c->fcur];
         a.inj = (n + 1 < n1 && n + 1 < c->g.nt) ? c->idh3 * c->wavelet[n + 1] : 0.0f;
-        a.seis_row = (record && c->nrec > 0 && n > c->g.tlag && n - c->g.tlag < c->g.nt)
-                         ? c->seis_tm + (long)(n - c->g.tlag) * c->nrec
+        a.seis_row = (record && c->nrec_total > 0 && n > c->g.tlag && n - c->g.tlag < c->g.nt)
+                         ? c->seis_tm + (long)(n - c->g.tlag) * c->nrec_total
                          : nullptr;
-        int rc = launch_step_timed(c, mode, t, a);
-        if (rc) return rc;
+        CHKRC(launch_step_timed(c, mode, t, a));
         c->fcur ^= 1;
     }
     return FWI_OK;
 }
 
-// Backward steps [b0, b1) from the current state (see fwi_backward_steps).
+// Backward steps [b0, b1) of every slot from the current state (receiver
+// data injected before each step b < nt, in receiver order per cell).
 int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     const int mode = kind == 0 ? fwi::BWD_RTM : fwi::BWD_FWI;
     const Tuning& t = c->tune[mode];
@@ -242,25 +321,29 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     a.illum = c->illum;
     if (b0 < b1 && b0 < nt)
         HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
-                                     c->data_tm + (long)(nt - 1 - b0) * c->nrec, c->idh3, c->st));
+                                     c->data_tm + (long)(nt - 1 - b0) * c->nrec_total, c->idh3, c->st));
     for (int b = b0; b < b1; ++b) {
         a.cur = c->f[1 - c->fcur];
         a.old = c->f[c->fcur];
         a.rcur = c->r[c->rcur];
         a.rold = c->r[1 - c->rcur];
-        int rc = launch_step_timed(c, mode, t, a);
-        if (rc) return rc;
+        CHKRC(launch_step_timed(c, mode, t, a));
         c->fcur ^= 1;
         c->rcur ^= 1;
         if (b + 1 < b1 && b + 1 < nt)
             HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
-                                         c->data_tm + (long)(nt - 2 - b) * c->nrec, c->idh3, c->st));
+                                         c->data_tm + (long)(nt - 2 - b) * c->nrec_total, c->idh3, c->st));
+    }
+    const int groups = c->nshots / t.sb;
+    if (b0 < b1 && groups > 1) {
+        HIPCHK(c, fwi::launch_fold_groups(kind == 0 ? c->image : c->grad, c->L.vol, groups, c->st));
+        HIPCHK(c, fwi::launch_fold_groups(c->illum, c->L.vol, groups, c->st));
     }
     return FWI_OK;
 }
 
-// Run `body` either directly on the stream or captured into a hipGraph,
-// bracketed by timing events.
+// Run `body` directly on the stream or captured in a hipGraph, bracketed by
+// timing events (recorded after graph instantiation).
 template <typename F>
 int timed_pass(fwi_ctx* c, int launches, F body) {
     c->kev_used = 0;
@@ -282,6 +365,8 @@ int timed_pass(fwi_ctx* c, int launches, F body) {
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (ei != hipSuccess) {
             (void)hipGraphDestroy(graph);
+            c->fcur = saved_f;
+            c->rcur = saved_r;
             return fail(c, FWI_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
         }
         HIPCHK(c, hipEventRecord(c->ev0, c->st));
@@ -292,8 +377,7 @@ int timed_pass(fwi_ctx* c, int launches, F body) {
         (void)hipGraphDestroy(graph);
     } else {
         HIPCHK(c, hipEventRecord(c->ev0, c->st));
-        int rc = body();
-        if (rc) return rc;
+        CHKRC(body());
         HIPCHK(c, hipEventRecord(c->ev1, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
     }
@@ -310,41 +394,66 @@ int timed_pass(fwi_ctx* c, int launches, F body) {
 }
 
 int check_ready_forward(fwi_ctx* c, int boundary) {
+    if (boundary != FWI_BOUNDARY_ABC && boundary != FWI_BOUNDARY_RBC) return fail(c, FWI_ERR_ARG, "boundary");
     if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
     if ((int)c->wavelet.size() != c->g.nt) return fail(c, FWI_ERR_STATE, "wavelet not set");
-    if (!c->have_src) return fail(c, FWI_ERR_STATE, "source not set");
+    for (int s = 0; s < c->nshots; ++s)
+        if (c->sx[s] < 0) return fail(c, FWI_ERR_STATE, "source not set for every shot slot");
     if (boundary == FWI_BOUNDARY_ABC && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
-    if (boundary != FWI_BOUNDARY_ABC && boundary != FWI_BOUNDARY_RBC) return fail(c, FWI_ERR_ARG, "boundary");
+    return sync_geometry(c);
+}
+
+// max/min/non-finite check of every slot's model.
+int check_model_stats(fwi_ctx* c) {
+    for (int s = 0; s < c->nshots; ++s) {
+        HIPCHK(c, fwi::launch_stats(c->vp + (long)s * c->L.vol, c->L, c->red_partial, c->red_result, c->st));
+        double h[3];
+        HIPCHK(c, hipMemcpyAsync(h, c->red_result, sizeof(h), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (h[2] > 0) return fail(c, FWI_ERR_NONFINITE, "model holds NaN/Inf");
+        const double cfl = std::max(std::fabs(h[0]), std::fabs(h[1])) * c->g.dt / c->g.dh;
+        if (cfl > fwi::CFL_MAX) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "CFL violated: vmax*dt/dh = %.6f > %.6f", cfl, fwi::CFL_MAX);
+            return fail(c, FWI_ERR_CFL, buf);
+        }
+    }
     return FWI_OK;
 }
 
-int check_model_stats(fwi_ctx* c) {
-    HIPCHK(c, fwi::launch_stats(c->vp, c->L, c->red_partial, c->red_result, c->st));
-    double h[3];
-    HIPCHK(c, hipMemcpyAsync(h, c->red_result, sizeof(h), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (h[2] > 0) return fail(c, FWI_ERR_NONFINITE, "model holds NaN/Inf");
-    const double cfl = std::max(std::fabs(h[0]), std::fabs(h[1])) * c->g.dt / c->g.dh;
-    if (cfl > fwi::CFL_MAX) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "CFL violated: vmax*dt/dh = %.6f > %.6f", cfl, fwi::CFL_MAX);
-        return fail(c, FWI_ERR_CFL, buf);
-    }
+// copy slot 0's model into the other slots
+int replicate_model(fwi_ctx* c) {
+    for (int s = 1; s < c->nshots; ++s)
+        HIPCHK(c, hipMemcpyAsync(c->vp + (long)s * c->L.vol, c->vp, c->L.vol * sizeof(float), hipMemcpyDeviceToDevice,
+                                 c->st));
     return FWI_OK;
 }
 
 const int kTileChoices[][2] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}, {4, 4}, {16, 1}};
 
+// Copy the columns of slot s out of a time-major [nt][nrec_total] matrix
+// into a compact [nt][nrec_s] one (to_slot = false) or back (true).
+int slot_columns(fwi_ctx* c, int s, float* tm, float* compact, bool to_slot) {
+    const int q0 = c->rec_start_h[s], ns = c->rec_start_h[s + 1] - q0;
+    const size_t wide = (size_t)c->nrec_total * sizeof(float), narrow = (size_t)ns * sizeof(float);
+    if (to_slot)
+        HIPCHK(c, hipMemcpy2DAsync(tm + q0, wide, compact, narrow, narrow, c->g.nt, hipMemcpyDeviceToDevice, c->st));
+    else
+        HIPCHK(c, hipMemcpy2DAsync(compact, narrow, tm + q0, wide, narrow, c->g.nt, hipMemcpyDeviceToDevice, c->st));
+    return FWI_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out) {
+int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out) {
     if (!grid || !out) return fail(nullptr, FWI_ERR_ARG, "NULL argument");
     *out = nullptr;
     const fwi_grid& g = *grid;
     if (g.nxx < 9 || g.nyy < 9 || g.nzz < 9 || g.nb < 0 || g.nt < 1 || g.tlag < 0 || !(g.dh > 0) || !(g.dt > 0))
         return fail(nullptr, FWI_ERR_ARG, "invalid grid");
+    if (nshots < 1) return fail(nullptr, FWI_ERR_ARG, "nshots must be >= 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, FWI_ERR_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return fail(nullptr, FWI_ERR_ARG, "device index out of range");
@@ -353,10 +462,17 @@ int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out) {
     fwi_ctx* c = new fwi_ctx();
     c->device = device;
     c->g = g;
+    c->nshots = nshots;
     c->L = fwi::make_layout(g.nxx, g.nyy, g.nzz);
     c->idh2 = 1.0f / (g.dh * g.dh);
     c->idh3 = 1.0f / (g.dh * g.dh * g.dh);
     c->dt2 = g.dt * g.dt;
+    c->sx.assign(nshots, -1);
+    c->sy.assign(nshots, -1);
+    c->sz.assign(nshots, -1);
+    c->rx.assign(nshots, {});
+    c->ry.assign(nshots, {});
+    c->rz.assign(nshots, {});
     auto bail = [&](int rc) {
         g_create_error = c->err;
         fwi_destroy(c);
@@ -367,22 +483,23 @@ int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out) {
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(c, FWI_ERR_HIP, "hipEventCreate failed"));
     int rc;
-    if ((rc = alloc_vol(c, &c->vp)) || (rc = alloc_vol(c, &c->f[0])) || (rc = alloc_vol(c, &c->f[1])))
+    if ((rc = alloc_slots(c, &c->vp)) || (rc = alloc_slots(c, &c->f[0])) || (rc = alloc_slots(c, &c->f[1])))
         return bail(rc);
     if (hipMalloc((void**)&c->staging, nvol_host(g) * sizeof(float)) != hipSuccess ||
         hipMalloc((void**)&c->red_partial, fwi::kReduceBlocks * 3 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->red_result, 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->src_off, sizeof(long)) != hipSuccess ||
-        hipMalloc((void**)&c->rec_start, 2 * sizeof(int)) != hipSuccess)
+        hipMalloc((void**)&c->src_off, nshots * sizeof(long)) != hipSuccess ||
+        hipMalloc((void**)&c->rec_start, (nshots + 1) * sizeof(int)) != hipSuccess)
         return bail(fail(c, FWI_ERR_HIP, "hipMalloc failed"));
-    int zero2[2] = {0, 0};
-    if (hipMemcpy(c->rec_start, zero2, sizeof zero2, hipMemcpyHostToDevice) != hipSuccess)
-        return bail(fail(c, FWI_ERR_HIP, "hipMemcpy failed"));
-    // default tiles: one wave row per thread row, 4 waves, 2 rows per thread
+    if (hipMemset(c->src_off, 0, nshots * sizeof(long)) != hipSuccess)
+        return bail(fail(c, FWI_ERR_HIP, "hipMemset failed"));
+    if ((rc = sync_geometry(c))) return bail(rc);
     for (auto& t : c->tune) t = Tuning{};
     *out = c;
     return FWI_OK;
 }
+
+int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out) { return fwi_create_batch(grid, 1, device, out); }
 
 int fwi_destroy(fwi_ctx* c) {
     if (!c) return FWI_OK;
@@ -409,6 +526,12 @@ const char* fwi_last_error(const fwi_ctx* c) { return c ? c->err.c_str() : g_cre
 int fwi_set_arith(fwi_ctx* c, int arith) {
     if (!c) return FWI_ERR_ARG;
     if (arith != FWI_ARITH_REFERENCE && arith != FWI_ARITH_FP32) return fail(c, FWI_ERR_ARG, "arith");
+    for (int k = 0; k < 4; ++k)
+        if (!fwi::step_config_supported(k, arith, c->tune[k])) {
+            c->tune[k].by = 4;
+            c->tune[k].rpt = 2;
+            c->tune[k].sb = 1;
+        }
     c->arith = arith;
     return FWI_OK;
 }
@@ -418,6 +541,7 @@ int fwi_set_model(fwi_ctx* c, const float* vp) {
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->staging, vp, nvol_host(c->g) * sizeof(float), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, fwi::launch_to_device(c->staging, c->vp, c->L, c->st));
+    CHKRC(replicate_model(c));
     int rc = check_model_stats(c);
     c->have_model = rc == FWI_OK;
     return rc;
@@ -426,33 +550,40 @@ int fwi_set_model(fwi_ctx* c, const float* vp) {
 int fwi_set_model_interior(fwi_ctx* c, const float* vp) {
     if (!c || !vp) return c ? fail(c, FWI_ERR_ARG, "NULL model") : FWI_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
-    const int nb = c->g.nb;
-    const long n = (long)(c->g.nxx - 2 * nb) * (c->g.nyy - 2 * nb) * (c->g.nzz - 2 * nb);
+    const long n = ninterior(c->g);
     if (n <= 0) return fail(c, FWI_ERR_ARG, "empty interior");
     HIPCHK(c, hipMemcpyAsync(c->staging, vp, n * sizeof(float), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, fwi::launch_expand(c->staging, c->vp, c->L, nb, c->st));
+    HIPCHK(c, fwi::launch_expand(c->staging, c->vp, c->L, c->g.nb, c->st));
+    CHKRC(replicate_model(c));
+    int rc = check_model_stats(c);
+    c->have_model = rc == FWI_OK;
+    return rc;
+}
+
+int fwi_shot_random_boundary(fwi_ctx* c, int slot, uint64_t seed, int shot, float amp, float vclamp) {
+    if (!c) return FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
+    if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
+    if (!(amp >= 0.0f && amp < 1.0f)) return fail(c, FWI_ERR_ARG, "amp must be in [0, 1)");
+    if (c->g.nb < 1) return fail(c, FWI_ERR_ARG, "random boundary needs nb >= 1");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, fwi::launch_random_boundary(c->vp + (long)slot * c->L.vol, c->L, c->g.nb, seed, shot, amp, vclamp,
+                                          c->st));
     int rc = check_model_stats(c);
     c->have_model = rc == FWI_OK;
     return rc;
 }
 
 int fwi_random_boundary(fwi_ctx* c, uint64_t seed, int shot, float amp, float vclamp) {
-    if (!c) return FWI_ERR_ARG;
-    if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
-    if (!(amp >= 0.0f && amp < 1.0f)) return fail(c, FWI_ERR_ARG, "amp must be in [0, 1)");
-    if (c->g.nb < 1) return fail(c, FWI_ERR_ARG, "random boundary needs nb >= 1");
-    HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, fwi::launch_random_boundary(c->vp, c->L, c->g.nb, seed, shot, amp, vclamp, c->st));
-    int rc = check_model_stats(c);
-    c->have_model = rc == FWI_OK;
-    return rc;
+    return fwi_shot_random_boundary(c, 0, seed, shot, amp, vclamp);
 }
 
 int fwi_set_wavelet(fwi_ctx* c, const float* w) {
     if (!c || !w) return c ? fail(c, FWI_ERR_ARG, "NULL wavelet") : FWI_ERR_ARG;
-    c->wavelet.assign(w, w + c->g.nt);
-    for (float v : c->wavelet)
-        if (!std::isfinite(v)) return fail(c, FWI_ERR_NONFINITE, "wavelet holds NaN/Inf");
+    std::vector<float> v(w, w + c->g.nt);
+    for (float x : v)
+        if (!std::isfinite(x)) return fail(c, FWI_ERR_NONFINITE, "wavelet holds NaN/Inf");
+    c->wavelet.swap(v);
     return FWI_OK;
 }
 
@@ -464,122 +595,117 @@ int fwi_set_dampers(fwi_ctx* c, const float* b1d) {
     c->b1d_h.assign(b1d, b1d + nb);
     if (!c->b1d) HIPCHK(c, hipMalloc((void**)&c->b1d, nb * sizeof(float)));
     HIPCHK(c, hipMemcpy(c->b1d, b1d, nb * sizeof(float), hipMemcpyHostToDevice));
-    return refresh_rec_damp(c);
-}
-
-int fwi_set_source(fwi_ctx* c, int sx, int sy, int sz) {
-    if (!c) return FWI_ERR_ARG;
-    if (!in_updated(c, sx, sy, sz)) return fail(c, FWI_ERR_RANGE, "source outside the updated interior");
-    HIPCHK(c, hipSetDevice(c->device));
-    c->sx = sx;
-    c->sy = sy;
-    c->sz = sz;
-    const long off = c->L.at(sx, sy, sz);
-    HIPCHK(c, hipMemcpy(c->src_off, &off, sizeof off, hipMemcpyHostToDevice));
-    c->have_src = true;
+    c->geom_dirty = true;  // receiver damping factors depend on the table
     return FWI_OK;
 }
 
-int fwi_set_receivers(fwi_ctx* c, int nrec, const int* rx, const int* ry, const int* rz) {
+int fwi_shot_set_source(fwi_ctx* c, int slot, int sx, int sy, int sz) {
     if (!c) return FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
+    if (!in_updated(c, sx, sy, sz)) return fail(c, FWI_ERR_RANGE, "source outside the updated interior");
+    HIPCHK(c, hipSetDevice(c->device));
+    c->sx[slot] = sx;
+    c->sy[slot] = sy;
+    c->sz[slot] = sz;
+    const long off = c->L.at(sx, sy, sz);
+    HIPCHK(c, hipMemcpy(c->src_off + slot, &off, sizeof off, hipMemcpyHostToDevice));
+    return FWI_OK;
+}
+
+int fwi_set_source(fwi_ctx* c, int sx, int sy, int sz) { return fwi_shot_set_source(c, 0, sx, sy, sz); }
+
+int fwi_shot_set_receivers(fwi_ctx* c, int slot, int nrec, const int* rx, const int* ry, const int* rz) {
+    if (!c) return FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
     if (nrec < 0 || (nrec > 0 && (!rx || !ry || !rz))) return fail(c, FWI_ERR_ARG, "bad receiver arrays");
     for (int q = 0; q < nrec; ++q)
         if (!in_updated(c, rx[q], ry[q], rz[q])) return fail(c, FWI_ERR_RANGE, "receiver outside the updated interior");
+    c->rx[slot].assign(rx, rx + nrec);
+    c->ry[slot].assign(ry, ry + nrec);
+    c->rz[slot].assign(rz, rz + nrec);
+    c->geom_dirty = true;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    for (void* p : {(void*)c->rec_off, (void*)c->rec_damp, (void*)c->seis_tm, (void*)c->data_tm, (void*)c->seis_tmp,
-                    (void*)c->cell_off, (void*)c->cell_start, (void*)c->cell_rec})
-        if (p) HIPCHK(c, hipFree(p));
-    c->rec_off = nullptr;
-    c->rec_damp = nullptr;
-    c->seis_tm = c->data_tm = c->seis_tmp = nullptr;
-    c->cell_off = nullptr;
-    c->cell_start = c->cell_rec = nullptr;
-    c->nrec = nrec;
-    c->rx.assign(rx, rx + nrec);
-    c->ry.assign(ry, ry + nrec);
-    c->rz.assign(rz, rz + nrec);
-    int se[2] = {0, nrec};
-    HIPCHK(c, hipMemcpy(c->rec_start, se, sizeof se, hipMemcpyHostToDevice));
-    c->ncells = 0;
-    if (nrec == 0) return FWI_OK;
-    std::vector<long> off(nrec);
-    for (int q = 0; q < nrec; ++q) off[q] = c->L.at(rx[q], ry[q], rz[q]);
-    // receivers grouped by cell, ascending receiver order inside a cell
-    std::vector<int> order(nrec);
-    for (int q = 0; q < nrec; ++q) order[q] = q;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return off[a] < off[b]; });
-    std::vector<long> coff;
-    std::vector<int> cstart;
-    for (int q = 0; q < nrec; ++q) {
-        if (q == 0 || off[order[q]] != off[order[q - 1]]) {
-            coff.push_back(off[order[q]]);
-            cstart.push_back(q);
-        }
-    }
-    cstart.push_back(nrec);
-    c->ncells = (int)coff.size();
-    const long ns = (long)c->g.nt * nrec;
-    HIPCHK(c, hipMalloc((void**)&c->rec_off, nrec * sizeof(long)));
-    HIPCHK(c, hipMalloc((void**)&c->cell_off, coff.size() * sizeof(long)));
-    HIPCHK(c, hipMalloc((void**)&c->cell_start, cstart.size() * sizeof(int)));
-    HIPCHK(c, hipMalloc((void**)&c->cell_rec, nrec * sizeof(int)));
-    HIPCHK(c, hipMalloc((void**)&c->seis_tm, ns * sizeof(float)));
-    HIPCHK(c, hipMalloc((void**)&c->data_tm, ns * sizeof(float)));
-    HIPCHK(c, hipMalloc((void**)&c->seis_tmp, ns * sizeof(float)));
-    HIPCHK(c, hipMemcpy(c->rec_off, off.data(), nrec * sizeof(long), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->cell_off, coff.data(), coff.size() * sizeof(long), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->cell_start, cstart.data(), cstart.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->cell_rec, order.data(), nrec * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->seis_tm, 0, ns * sizeof(float)));
-    HIPCHK(c, hipMemset(c->data_tm, 0, ns * sizeof(float)));
-    return refresh_rec_damp(c);
+    return sync_geometry(c);
+}
+
+int fwi_set_receivers(fwi_ctx* c, int nrec, const int* rx, const int* ry, const int* rz) {
+    return fwi_shot_set_receivers(c, 0, nrec, rx, ry, rz);
 }
 
 int fwi_forward_steps(fwi_ctx* c, int boundary, int n0, int n1) {
     if (!c) return FWI_ERR_ARG;
-    int rc = check_ready_forward(c, boundary);
-    if (rc) return rc;
-    if (n0 < 0 || n1 < n0 || n1 > c->g.nt + c->g.tlag) return fail(c, FWI_ERR_ARG, "step range");
     HIPCHK(c, hipSetDevice(c->device));
-    rc = timed_pass(c, n1 - n0, [&] { return run_forward(c, boundary, n0, n1); });
+    CHKRC(check_ready_forward(c, boundary));
+    if (n0 < 0 || n1 < n0 || n1 > c->g.nt + c->g.tlag) return fail(c, FWI_ERR_ARG, "step range");
+    int rc = timed_pass(c, n1 - n0, [&] { return run_forward(c, boundary, n0, n1); });
     c->fwd_done = 0;
     return rc;
 }
 
 int fwi_forward(fwi_ctx* c, int boundary) {
     if (!c) return FWI_ERR_ARG;
-    int rc = check_ready_forward(c, boundary);
-    if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemsetAsync(c->f[0], 0, c->L.vol * sizeof(float), c->st));
-    HIPCHK(c, hipMemsetAsync(c->f[1], 0, c->L.vol * sizeof(float), c->st));
-    if (c->nrec > 0) HIPCHK(c, hipMemsetAsync(c->seis_tm, 0, (long)c->g.nt * c->nrec * sizeof(float), c->st));
+    CHKRC(check_ready_forward(c, boundary));
+    const long vols = c->L.vol * c->nshots;
+    HIPCHK(c, hipMemsetAsync(c->f[0], 0, vols * sizeof(float), c->st));
+    HIPCHK(c, hipMemsetAsync(c->f[1], 0, vols * sizeof(float), c->st));
+    if (c->nrec_total > 0)
+        HIPCHK(c, hipMemsetAsync(c->seis_tm, 0, (long)c->g.nt * c->nrec_total * sizeof(float), c->st));
     c->fcur = 0;
     const int N = c->g.nt + c->g.tlag;
-    rc = timed_pass(c, N, [&] { return run_forward(c, boundary, 0, N); });
+    int rc = timed_pass(c, N, [&] { return run_forward(c, boundary, 0, N); });
     c->fwd_done = rc == FWI_OK ? 1 : 0;
     return rc;
 }
 
-int fwi_get_seismogram(fwi_ctx* c, float* out) {
+int fwi_shot_get_seismogram(fwi_ctx* c, int slot, float* out) {
     if (!c || !out) return c ? fail(c, FWI_ERR_ARG, "NULL output") : FWI_ERR_ARG;
-    if (c->nrec == 0) return FWI_OK;
+    CHKRC(check_slot(c, slot));
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, fwi::launch_transpose2d(c->seis_tm, c->seis_tmp, c->g.nt, c->nrec, c->st));
-    HIPCHK(c, hipMemcpyAsync(out, c->seis_tmp, (long)c->g.nt * c->nrec * sizeof(float), hipMemcpyDeviceToHost,
-                             c->st));
+    CHKRC(sync_geometry(c));
+    const int ns = c->rec_start_h[slot + 1] - c->rec_start_h[slot];
+    if (ns == 0) return FWI_OK;
+    float* compact = c->seis_tmp;
+    float* tr = c->seis_tmp + (long)c->g.nt * c->nrec_total;
+    CHKRC(slot_columns(c, slot, c->seis_tm, compact, false));
+    HIPCHK(c, fwi::launch_transpose2d(compact, tr, c->g.nt, ns, c->st));
+    HIPCHK(c, hipMemcpyAsync(out, tr, (long)c->g.nt * ns * sizeof(float), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return FWI_OK;
 }
 
-int fwi_set_backward_data(fwi_ctx* c, const float* data) {
+int fwi_get_seismogram(fwi_ctx* c, float* out) { return fwi_shot_get_seismogram(c, 0, out); }
+
+int fwi_shot_set_backward_data(fwi_ctx* c, int slot, const float* data) {
     if (!c || !data) return c ? fail(c, FWI_ERR_ARG, "NULL data") : FWI_ERR_ARG;
-    if (c->nrec == 0) return FWI_OK;
+    CHKRC(check_slot(c, slot));
     HIPCHK(c, hipSetDevice(c->device));
-    const long ns = (long)c->g.nt * c->nrec;
-    HIPCHK(c, hipMemcpyAsync(c->seis_tmp, data, ns * sizeof(float), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, fwi::launch_transpose2d(c->seis_tmp, c->data_tm, c->nrec, c->g.nt, c->st));
+    CHKRC(sync_geometry(c));
+    const int ns = c->rec_start_h[slot + 1] - c->rec_start_h[slot];
+    if (ns == 0) return FWI_OK;
+    float* up = c->seis_tmp;
+    float* compact = c->seis_tmp + (long)c->g.nt * c->nrec_total;
+    HIPCHK(c, hipMemcpyAsync(up, data, (long)c->g.nt * ns * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, fwi::launch_transpose2d(up, compact, ns, c->g.nt, c->st));
+    CHKRC(slot_columns(c, slot, c->data_tm, compact, true));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FWI_OK;
+}
+
+int fwi_set_backward_data(fwi_ctx* c, const float* data) { return fwi_shot_set_backward_data(c, 0, data); }
+
+int fwi_shot_get_backward_data(fwi_ctx* c, int slot, float* out) {
+    if (!c || !out) return c ? fail(c, FWI_ERR_ARG, "NULL output") : FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
+    HIPCHK(c, hipSetDevice(c->device));
+    CHKRC(sync_geometry(c));
+    const int ns = c->rec_start_h[slot + 1] - c->rec_start_h[slot];
+    if (ns == 0) return FWI_OK;
+    float* compact = c->seis_tmp;
+    float* tr = c->seis_tmp + (long)c->g.nt * c->nrec_total;
+    CHKRC(slot_columns(c, slot, c->data_tm, compact, false));
+    HIPCHK(c, fwi::launch_transpose2d(compact, tr, c->g.nt, ns, c->st));
+    HIPCHK(c, hipMemcpyAsync(out, tr, (long)c->g.nt * ns * sizeof(float), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return FWI_OK;
 }
@@ -590,8 +716,8 @@ int fwi_backward_steps(fwi_ctx* c, int kind, int b0, int b1) {
     if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
     if (b0 < 0 || b1 < b0 || b1 > c->g.nt + c->g.tlag) return fail(c, FWI_ERR_ARG, "step range");
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_backward_buffers(c, kind);
-    if (rc) return rc;
+    CHKRC(sync_geometry(c));
+    CHKRC(ensure_backward_buffers(c, kind));
     return timed_pass(c, b1 - b0, [&] { return run_backward(c, kind, b0, b1); });
 }
 
@@ -599,13 +725,14 @@ static int full_backward(fwi_ctx* c, int kind) {
     if (!c) return FWI_ERR_ARG;
     if (!c->fwd_done) return fail(c, FWI_ERR_STATE, "backward pass needs a completed forward pass");
     HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_backward_buffers(c, kind);
-    if (rc) return rc;
-    HIPCHK(c, hipMemsetAsync(c->r[0], 0, c->L.vol * sizeof(float), c->st));
-    HIPCHK(c, hipMemsetAsync(c->r[1], 0, c->L.vol * sizeof(float), c->st));
+    CHKRC(sync_geometry(c));
+    CHKRC(ensure_backward_buffers(c, kind));
+    const long vols = c->L.vol * c->nshots;
+    HIPCHK(c, hipMemsetAsync(c->r[0], 0, vols * sizeof(float), c->st));
+    HIPCHK(c, hipMemsetAsync(c->r[1], 0, vols * sizeof(float), c->st));
     c->rcur = 0;
     const int N = c->g.nt + c->g.tlag;
-    rc = timed_pass(c, N, [&] { return run_backward(c, kind, 0, N); });
+    int rc = timed_pass(c, N, [&] { return run_backward(c, kind, 0, N); });
     c->fwd_done = 0;
     return rc;
 }
@@ -617,18 +744,19 @@ int fwi_zero_volume(fwi_ctx* c, int which) {
     if (!c) return FWI_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
-    float* p = vol_ptr(c, which, true, &rc);
+    float* p = vol_ptr(c, 0, which, true, &rc);
     if (!p) return rc;
-    HIPCHK(c, hipMemsetAsync(p, 0, c->L.vol * sizeof(float), c->st));
+    HIPCHK(c, hipMemsetAsync(p, 0, c->L.vol * c->nshots * sizeof(float), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return FWI_OK;
 }
 
-int fwi_get_volume(fwi_ctx* c, int which, float* out) {
+int fwi_shot_get_volume(fwi_ctx* c, int slot, int which, float* out) {
     if (!c || !out) return c ? fail(c, FWI_ERR_ARG, "NULL output") : FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
-    float* p = vol_ptr(c, which, true, &rc);
+    float* p = vol_ptr(c, slot, which, true, &rc);
     if (!p) return rc;
     HIPCHK(c, fwi::launch_to_host(p, c->staging, c->L, 0, c->st));
     HIPCHK(c, hipMemcpyAsync(out, c->staging, nvol_host(c->g) * sizeof(float), hipMemcpyDeviceToHost, c->st));
@@ -636,12 +764,14 @@ int fwi_get_volume(fwi_ctx* c, int which, float* out) {
     return FWI_OK;
 }
 
+int fwi_get_volume(fwi_ctx* c, int which, float* out) { return fwi_shot_get_volume(c, 0, which, out); }
+
 int fwi_set_volume(fwi_ctx* c, int which, const float* in) {
     if (!c || !in) return c ? fail(c, FWI_ERR_ARG, "NULL input") : FWI_ERR_ARG;
     if (which == FWI_VOL_MODEL) return fwi_set_model(c, in);
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
-    float* p = vol_ptr(c, which, true, &rc);
+    float* p = vol_ptr(c, 0, which, true, &rc);
     if (!p) return rc;
     HIPCHK(c, hipMemcpyAsync(c->staging, in, nvol_host(c->g) * sizeof(float), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, fwi::launch_to_device(c->staging, p, c->L, c->st));
@@ -652,7 +782,7 @@ int fwi_set_volume(fwi_ctx* c, int which, const float* in) {
 
 static int export_interior(fwi_ctx* c, const float* num, int zfirst, float* out, float* gmax) {
     const int nb = c->g.nb;
-    const long n = (long)(c->g.nxx - 2 * nb) * (c->g.nyy - 2 * nb) * (c->g.nzz - 2 * nb);
+    const long n = ninterior(c->g);
     HIPCHK(c, fwi::launch_export_image(num, c->illum, c->staging, c->L, nb, c->idh2, zfirst, c->st));
     if (gmax) {
         HIPCHK(c, fwi::launch_absmax(c->staging, n, c->red_partial, c->red_result, c->st));
@@ -686,21 +816,29 @@ int fwi_export_gradient(fwi_ctx* c, float zmask, float* out, float* gmax) {
     return export_interior(c, c->grad, zfirst, out, gmax);
 }
 
-int fwi_residual(fwi_ctx* c, const float* obs, double* sum_sq) {
+int fwi_shot_residual(fwi_ctx* c, int slot, const float* obs, double* sum_sq) {
     if (!c || !obs || !sum_sq) return c ? fail(c, FWI_ERR_ARG, "NULL argument") : FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
     *sum_sq = 0.0;
-    if (c->nrec == 0) return FWI_OK;
     HIPCHK(c, hipSetDevice(c->device));
-    const long ns = (long)c->g.nt * c->nrec;
-    // obs (t + r*nt) -> time-major in seis_tmp; residual written to data_tm
-    HIPCHK(c, hipMemcpyAsync(c->data_tm, obs, ns * sizeof(float), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, fwi::launch_transpose2d(c->data_tm, c->seis_tmp, c->nrec, c->g.nt, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->data_tm, c->seis_tm, ns * sizeof(float), hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, fwi::launch_residual(c->seis_tmp, c->data_tm, ns, c->red_partial, c->red_result, c->st));
+    CHKRC(sync_geometry(c));
+    const int ns = c->rec_start_h[slot + 1] - c->rec_start_h[slot];
+    if (ns == 0) return FWI_OK;
+    const long n = (long)c->g.nt * ns;
+    float* obs_tm = c->seis_tmp;
+    float* cal = c->seis_tmp + (long)c->g.nt * c->nrec_total;
+    // obs (t + r*nt) -> [t][r] in obs_tm via cal as upload buffer
+    HIPCHK(c, hipMemcpyAsync(cal, obs, n * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, fwi::launch_transpose2d(cal, obs_tm, ns, c->g.nt, c->st));
+    CHKRC(slot_columns(c, slot, c->seis_tm, cal, false));
+    HIPCHK(c, fwi::launch_residual(obs_tm, cal, n, c->red_partial, c->red_result, c->st));
+    CHKRC(slot_columns(c, slot, c->data_tm, cal, true));
     HIPCHK(c, hipMemcpyAsync(sum_sq, c->red_result, sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return FWI_OK;
 }
+
+int fwi_residual(fwi_ctx* c, const float* obs, double* sum_sq) { return fwi_shot_residual(c, 0, obs, sum_sq); }
 
 int fwi_get_tuning(fwi_ctx* c, int kind, int* by, int* rpt, int* zchunks) {
     if (!c || kind < 0 || kind > 3 || !by || !rpt || !zchunks) return FWI_ERR_ARG;
@@ -713,23 +851,42 @@ int fwi_get_tuning(fwi_ctx* c, int kind, int* by, int* rpt, int* zchunks) {
 int fwi_set_tuning(fwi_ctx* c, int kind, int by, int rpt, int zchunks) {
     if (!c) return FWI_ERR_ARG;
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
-    Tuning t;
+    Tuning t = c->tune[kind];
     t.by = by;
     t.rpt = rpt;
     t.zchunks = zchunks;
-    if (zchunks < 1 || zchunks > (c->g.nzz - 8) || !fwi::step_config_supported(kind, c->arith, t, 1))
+    if (zchunks < 1 || zchunks > (c->g.nzz - 8) || !fwi::step_config_supported(kind, c->arith, t))
         return fail(c, FWI_ERR_ARG, "unsupported tile configuration");
     c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_set_shots_per_block(fwi_ctx* c, int kind, int sb) {
+    if (!c) return FWI_ERR_ARG;
+    if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
+    Tuning t = c->tune[kind];
+    t.sb = sb;
+    if (sb < 1 || c->nshots % sb != 0 || !fwi::step_config_supported(kind, c->arith, t))
+        return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
+    c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_get_shots_per_block(fwi_ctx* c, int kind, int* sb) {
+    if (!c || kind < 0 || kind > 3 || !sb) return FWI_ERR_ARG;
+    *sb = c->tune[kind].sb;
     return FWI_OK;
 }
 
 int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     if (!c) return FWI_ERR_ARG;
     if (kind < 0 || kind > 3 || steps < 1) return fail(c, FWI_ERR_ARG, "kind/steps");
+    if (kind == 1 && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
     HIPCHK(c, hipSetDevice(c->device));
-    char key[160];
-    std::snprintf(key, sizeof key, "step kind=%d arith=%d nxx=%d nyy=%d nzz=%d sb=1", kind, c->arith, c->g.nxx,
-                  c->g.nyy, c->g.nzz);
+    CHKRC(sync_geometry(c));
+    char key[200];
+    std::snprintf(key, sizeof key, "step kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith, c->g.nxx,
+                  c->g.nyy, c->g.nzz, c->nshots);
     if (cache_path) {
         std::ifstream in(cache_path);
         std::string line;
@@ -737,56 +894,67 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
             const size_t bar = line.find('|');
             if (bar == std::string::npos || line.substr(0, bar) != key) continue;
             std::istringstream ss(line.substr(bar + 1));
-            int by, rpt, zc;
-            if (ss >> by >> rpt >> zc && fwi_set_tuning(c, kind, by, rpt, zc) == FWI_OK) return FWI_OK;
-        }
-    }
-    const bool bwd = kind >= 2;
-    if (bwd) {
-        int rc = ensure_backward_buffers(c, kind - 2);
-        if (rc) return rc;
-    }
-    const Tuning saved = c->tune[kind];
-    const int saved_graph = c->use_graph;
-    c->use_graph = false;
-    double best = 1e300;
-    Tuning best_t = saved;
-    const int zc_choices[] = {1, 2, 4, 8};
-    for (auto& tc : kTileChoices) {
-        for (int zc : zc_choices) {
             Tuning t;
-            t.by = tc[0];
-            t.rpt = tc[1];
-            t.zchunks = zc;
-            if (!fwi::step_config_supported(kind, c->arith, t, 1) || zc > c->g.nzz - 8) continue;
-            c->tune[kind] = t;
-            auto body = [&](int nsteps) {
-                // trial steps past nt: no injection, no recording, no data reads
-                if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
-                return run_forward(c, kind == 1 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC, c->g.nt, c->g.nt + nsteps,
-                                   false);
-            };
-            if (kind == 1 && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
-            int rc = timed_pass(c, 1, [&] { return body(1); });
-            if (rc) return rc;
-            rc = timed_pass(c, steps, [&] { return body(steps); });
-            if (rc) return rc;
-            if (c->last_ms < best) {
-                best = c->last_ms;
-                best_t = t;
+            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb && c->nshots % t.sb == 0 &&
+                fwi::step_config_supported(kind, c->arith, t)) {
+                c->tune[kind] = t;
+                return FWI_OK;
             }
         }
     }
-    c->tune[kind] = best_t;
+    const bool bwd = kind >= 2;
+    if (bwd) CHKRC(ensure_backward_buffers(c, kind - 2));
+    const Tuning saved = c->tune[kind];
+    const bool saved_graph = c->use_graph, saved_kt = c->ktime;
+    c->use_graph = false;
+    c->ktime = false;
+    double best = 1e300;
+    Tuning best_t = saved;
+    const int zc_choices[] = {1, 2, 4, 8, 16};
+    const int sb_choices[] = {1, 2, 4};
+    int rc = FWI_OK;
+    for (int sb : sb_choices) {
+        if (c->nshots % sb != 0) continue;
+        for (auto& tc : kTileChoices) {
+            for (int zc : zc_choices) {
+                Tuning t;
+                t.by = tc[0];
+                t.rpt = tc[1];
+                t.zchunks = zc;
+                t.sb = sb;
+                if (!fwi::step_config_supported(kind, c->arith, t) || zc > c->g.nzz - 8) continue;
+                c->tune[kind] = t;
+                // trial steps past nt: no injection, no recording, no data reads
+                auto body = [&](int nsteps) {
+                    if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
+                    return run_forward(c, kind == 1 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC, c->g.nt,
+                                       c->g.nt + nsteps, false);
+                };
+                if ((rc = timed_pass(c, 1, [&] { return body(1); }))) break;
+                if ((rc = timed_pass(c, steps, [&] { return body(steps); }))) break;
+                if (c->last_ms < best) {
+                    best = c->last_ms;
+                    best_t = t;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    c->tune[kind] = rc ? saved : best_t;
     c->use_graph = saved_graph;
+    c->ktime = saved_kt;
     // the trial steps ran on the live buffers: clear them
+    const long vols = c->L.vol * c->nshots;
     for (float* p : {c->f[0], c->f[1], c->r[0], c->r[1], c->image, c->illum, c->grad})
-        if (p) HIPCHK(c, hipMemsetAsync(p, 0, c->L.vol * sizeof(float), c->st));
+        if (p) HIPCHK(c, hipMemsetAsync(p, 0, vols * sizeof(float), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->fwd_done = 0;
+    if (rc) return rc;
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
-        out << key << "|" << best_t.by << " " << best_t.rpt << " " << best_t.zchunks << " " << best << "\n";
+        out << key << "|" << best_t.by << " " << best_t.rpt << " " << best_t.zchunks << " " << best_t.sb << " "
+            << best << "\n";
     }
     return FWI_OK;
 }

@@ -54,6 +54,7 @@ struct Tuning {
     int by = 4;       // waves (rows of 64 lanes) per block
     int rpt = 2;      // tile rows per thread (register blocking in y)
     int zchunks = 1;  // blocks along z per (x, y) tile column
+    int sb = 1;       // shots per block (batched launches)
 };
 
 // Arguments of one step launch.  The fields of shot s live at base + s*vol.
@@ -87,8 +88,8 @@ struct StepArgs {
 };
 
 // Host-side launchers (kernels_step.hip).
-hipError_t launch_step(int mode, int arith, const Tuning& t, int sb, const StepArgs& a, hipStream_t st);
-bool step_config_supported(int mode, int arith, const Tuning& t, int sb);
+hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st);
+bool step_config_supported(int mode, int arith, const Tuning& t);
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
 
 // Host-side launchers (kernels_aux.hip).  "host layout" buffers here are
@@ -98,7 +99,8 @@ hipError_t launch_to_host(const float* dev, float* host_layout, const Layout& L,
 hipError_t launch_expand(const float* interior_host_layout, float* dev, const Layout& L, int nb, hipStream_t st);
 hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t seed, int shot, float amp,
                                   float vclamp, hipStream_t st);
-hipError_t launch_add_point(float* dev, long off, float value, hipStream_t st);
+hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, float value, hipStream_t st);
+hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st);
 hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec,
                          int ncells, const float* data_row, float idh3, hipStream_t st);
 hipError_t launch_export_image(const float* image, const float* illum, float* out, const Layout& L, int nb,

@@ -101,8 +101,28 @@ __global__ void __launch_bounds__(256) random_boundary_kernel(float* __restrict_
     dev[L.at(x, y, z)] = val;
 }
 
-__global__ void add_point_kernel(float* dev, long off, float value) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) dev[off] = dev[off] + value;
+// Source injection of the first step for every shot of a launch
+// (compute_pressure's atomicAdd, modeling.cu:347-348): shot s's field starts
+// at s*sstride and its source cell is off[s].
+__global__ void add_points_kernel(float* dev, const long* __restrict__ off, int n, long sstride, float value) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        float* p = dev + (long)s * sstride + off[s];
+        *p = *p + value;
+    }
+}
+
+// Fold shot-group partial accumulators into group 0 (in group order) and
+// clear the others: image/illumination of a batched backward pass.
+__global__ void __launch_bounds__(256) fold_groups_kernel(float* __restrict__ v, long vol, int groups) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < vol; q += (long)gridDim.x * 256) {
+        float acc = v[q];
+        for (int g = 1; g < groups; ++g) {
+            acc = acc + v[(long)g * vol + q];
+            v[(long)g * vol + q] = 0.0f;
+        }
+        v[q] = acc;
+    }
 }
 
 // inject_seismogram / inject_residuo (migration.cu:119-132,
@@ -290,8 +310,14 @@ hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_add_point(float* dev, long off, float value, hipStream_t st) {
-    add_point_kernel<<<1, 64, 0, st>>>(dev, off, value);
+hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, float value, hipStream_t st) {
+    add_points_kernel<<<(n + 63) / 64, 64, 0, st>>>(dev, off, n, sstride, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st) {
+    if (groups <= 1) return hipSuccess;
+    fold_groups_kernel<<<4096, 256, 0, st>>>(v, vol, groups);
     return hipGetLastError();
 }
 

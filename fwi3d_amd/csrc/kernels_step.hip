@@ -75,21 +75,29 @@ __device__ __forceinline__ float d2(float c, float m1, float p1, float m2, float
 // get_boundary_damper (modeling.cu:395-520) from the 1-D table only: the
 // reference's b2d/b3d entries are fl(b1[a]+b1[b]) - 1 and
 // ((fl(b1[i]+b1[j]) + fl(b1[j]+b1[k])) + fl(b1[i]+b1[k])) - 5 (modeling.cu:
-// 101-127), recomputed here with the same float operations.
-__device__ __forceinline__ float damper(const float* __restrict__ b1d, int nb, int x, int y, int z, const Layout& L) {
-    const int di = z < nb ? z : (z >= L.nz - nb ? L.nz - 1 - z : -1);  // reference i (z)
-    const int dj = x < nb ? x : (x >= L.nx - nb ? L.nx - 1 - x : -1);  // reference j (x)
-    const int dk = y < nb ? y : (y >= L.ny - nb ? L.ny - 1 - y : -1);  // reference k (y)
-    const int cnt = (di >= 0) + (dj >= 0) + (dk >= 0);
+// 101-127), recomputed here with the same float operations.  The lookup is
+// split per axis: the x and y parts are fixed while a thread marches along
+// z, and the z part is uniform over a block.
+struct DampAxis {
+    float b;  // b1d[d] when the cell is within nb of a face on this axis
+    bool on;
+};
+
+__device__ __forceinline__ DampAxis damp_axis(const float* __restrict__ b1d, int nb, int q, int n) {
+    const int d = q < nb ? q : (q >= n - nb ? n - 1 - q : -1);
+    return {d >= 0 ? b1d[d] : 0.0f, d >= 0};
+}
+
+__device__ __forceinline__ float damper(DampAxis i, DampAxis j, DampAxis k) {
+    const int cnt = (int)i.on + (int)j.on + (int)k.on;
     if (cnt == 0) return 1.0f;
-    if (cnt == 1) return b1d[di >= 0 ? di : (dj >= 0 ? dj : dk)];
+    if (cnt == 1) return i.on ? i.b : (j.on ? j.b : k.b);
     if (cnt == 2) {
-        const int a = di >= 0 ? di : dj;
-        const int b = dk >= 0 ? dk : dj;
-        return (b1d[a] + b1d[b]) - 1.0f;
+        const float a = i.on ? i.b : j.b;
+        const float b = k.on ? k.b : j.b;
+        return (a + b) - 1.0f;
     }
-    const float bi = b1d[di], bj = b1d[dj], bk = b1d[dk];
-    const float t1 = bi + bj, t2 = bj + bk, t3 = bi + bk;
+    const float t1 = i.b + j.b, t2 = j.b + k.b, t3 = i.b + k.b;
     return ((t1 + t2) + t3) - 5.0f;
 }
 
@@ -171,6 +179,13 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             hy_y[m] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
             hy_row[m] = h < 4 ? h : TY + h;
             hy_ok[m] = h < 8 && xld && hy_y[m] < L.ny;
+        }
+
+        DampAxis dmp_x{0.0f, false}, dmp_y[RPT];
+        if constexpr (ABC) {
+            dmp_x = damp_axis(a.b1d, a.nb, x, L.nx);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) dmp_y[r] = damp_axis(a.b1d, a.nb, yr[r], L.ny);
         }
 
         const float* fld[SB][NF];
@@ -271,6 +286,8 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             }
 
             // 2. compute plane z
+            DampAxis dmp_z{0.0f, false};
+            if constexpr (ABC) dmp_z = damp_axis(a.b1d, a.nb, z, L.nz);
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 if (!act[r]) continue;
@@ -295,7 +312,7 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                         float oo = o[s][r];
                         float dmp = 1.0f;
                         if constexpr (ABC) {
-                            dmp = damper(a.b1d, a.nb, x, yr[r], z, L);
+                            dmp = damper(dmp_z, dmp_x, dmp_y[r]);
                             oo = oo * dmp;
                         }
                         float nv = K * ((dxx + dyy) + dzz);
@@ -392,28 +409,30 @@ struct Entry {
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB)
 
+#define FWI_BATCH(M)                                                                                     \
+    FWI_E(M, ARITH_REF, 4, 1, 2), FWI_E(M, ARITH_REF, 4, 2, 2), FWI_E(M, ARITH_REF, 8, 1, 2),            \
+        FWI_E(M, ARITH_REF, 16, 1, 2), FWI_E(M, ARITH_REF, 4, 1, 4), FWI_E(M, ARITH_REF, 8, 1, 4),        \
+        FWI_E(M, ARITH_REF, 16, 1, 4)
+
 const Entry kTable[] = {
     FWI_TILES(FWD_RBC, ARITH_REF, 1), FWI_TILES(FWD_ABC, ARITH_REF, 1),
     FWI_TILES(BWD_RTM, ARITH_REF, 1), FWI_TILES(BWD_FWI, ARITH_REF, 1),
     FWI_TILES(FWD_RBC, ARITH_FP32, 1), FWI_TILES(FWD_ABC, ARITH_FP32, 1),
     FWI_TILES(BWD_RTM, ARITH_FP32, 1), FWI_TILES(BWD_FWI, ARITH_FP32, 1),
     // batched: shots per block 2 and 4
-    FWI_E(FWD_RBC, ARITH_REF, 4, 1, 2), FWI_E(FWD_RBC, ARITH_REF, 4, 2, 2), FWI_E(FWD_RBC, ARITH_REF, 8, 1, 2),
-    FWI_E(FWD_RBC, ARITH_REF, 4, 1, 4), FWI_E(FWD_RBC, ARITH_REF, 8, 1, 4),
-    FWI_E(BWD_RTM, ARITH_REF, 4, 1, 2), FWI_E(BWD_RTM, ARITH_REF, 4, 2, 2), FWI_E(BWD_RTM, ARITH_REF, 8, 1, 2),
-    FWI_E(BWD_RTM, ARITH_REF, 4, 1, 4), FWI_E(BWD_RTM, ARITH_REF, 8, 1, 4),
+    FWI_BATCH(FWD_RBC), FWI_BATCH(FWD_ABC), FWI_BATCH(BWD_RTM), FWI_BATCH(BWD_FWI),
 };
 
-const Entry* find(int mode, int arith, const Tuning& t, int sb) {
+const Entry* find(int mode, int arith, const Tuning& t) {
     for (const Entry& e : kTable)
-        if (e.mode == mode && e.arith == arith && e.by == t.by && e.rpt == t.rpt && e.sb == sb) return &e;
+        if (e.mode == mode && e.arith == arith && e.by == t.by && e.rpt == t.rpt && e.sb == t.sb) return &e;
     return nullptr;
 }
 
 }  // namespace
 
-bool step_config_supported(int mode, int arith, const Tuning& t, int sb) {
-    return find(mode, arith, t, sb) != nullptr && t.zchunks >= 1;
+bool step_config_supported(int mode, int arith, const Tuning& t) {
+    return find(mode, arith, t) != nullptr && t.zchunks >= 1;
 }
 
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block) {
@@ -432,12 +451,12 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     block = dim3(64, t.by);
 }
 
-hipError_t launch_step(int mode, int arith, const Tuning& t, int sb, const StepArgs& a, hipStream_t st) {
-    const Entry* e = find(mode, arith, t, sb);
-    if (!e) return hipErrorInvalidConfiguration;
+hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
+    const Entry* e = find(mode, arith, t);
+    if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
     StepArgs args = a;
     dim3 grid, block;
-    step_grid(a.L, t, a.nshots, sb, args, grid, block);
+    step_grid(a.L, t, a.nshots, t.sb, args, grid, block);
     return e->fn(args, grid, st);
 }
 

@@ -1,0 +1,220 @@
+// inversion.cpp -- see inversion.hpp.
+#include "inversion.hpp"
+
+#include <cmath>
+#include <fstream>
+#include <iostream>
+
+#include "admin.hpp"
+
+Inversion::~Inversion() {
+    if (rbc) fwi_destroy(rbc);
+}
+
+// inversion.cu:3-53
+void Inversion::set_parameters() {
+    title = "Full Waveform Inversion";
+    set_main_parameters();
+    set_build_parameters();
+    set_abc_dampers();
+    abc_nb = nb;
+    set_wavelet();
+    set_geometry();
+    set_properties();
+    set_seismograms();
+
+    rbc_ratio = std::stof(catch_parameter("inv_rbc_ratio", parameters));
+    rbc_varVp = std::stof(catch_parameter("inv_rbc_varVp", parameters));
+    rbc_length = std::stof(catch_parameter("inv_rbc_length", parameters));
+    rbc_nb = (int)(rbc_length / dh) + 1;
+
+    zmask = std::stof(catch_parameter("depth_mask", parameters));
+    input_folder = catch_parameter("inv_input_folder", parameters);
+    input_prefix = catch_parameter("inv_input_prefix", parameters);
+    output_folder = catch_parameter("inv_output_folder", parameters);
+    residuo_folder = catch_parameter("inv_residuo_folder", parameters);
+    max_iteration = std::stoi(catch_parameter("max_iteration", parameters));
+    iteration = 0;
+
+    create_context(rbc_nb, &rbc);
+    create_context(abc_nb, &ctx);  // leaves nb / nxx / nyy / nzz at the ABC grid
+    check(fwi_set_dampers(ctx, b1d.data()), "fwi_set_dampers");
+
+    obs_data.assign((size_t)nt * geometry->nrec, 0.0f);
+    residual.assign((size_t)nt * geometry->nrec, 0.0f);
+    gradient.assign(nPoints, 0.0f);
+    trial.assign(nPoints, 0.0f);
+    set_initial_model();
+}
+
+// inversion.cu:105-113: slowness squared of the starting model.
+void Inversion::set_initial_model() {
+    m.assign(nPoints, 0.0f);
+    for (long q = 0; q < nPoints; q++) m[q] = 1.0f / (model[q] * model[q]);
+}
+
+// inversion.cu:172-176
+void Inversion::set_obs_data(int shot) {
+    const std::string data_file = input_folder + input_prefix + std::to_string(shot + 1) + ".bin";
+    import_binary_float(data_file, obs_data.data(), (long)nt * geometry->nrec);
+}
+
+// Modeling::initialization on context c (boundary cells nbc) for the shots
+// [first, first + count); unused slots repeat the first shot.
+void Inversion::place_shots(fwi_ctx* c, int nbc, int first, int count) {
+    auto idx = [&](float v) { return (int)((v + 0.5f * dh) / dh) + nbc; };
+    std::vector<int> rx(geometry->nrec), ry(geometry->nrec), rz(geometry->nrec);
+    for (int r = 0; r < geometry->nrec; r++) {
+        rx[r] = idx(geometry->xrec[r]);
+        ry[r] = idx(geometry->yrec[r]);
+        rz[r] = idx(geometry->zrec[r]);
+    }
+    for (int slot = 0; slot < batch; slot++) {
+        const int shot = first + (slot < count ? slot : 0);
+        check(fwi_shot_set_source(c, slot, idx(geometry->xsrc[shot]), idx(geometry->ysrc[shot]),
+                                  idx(geometry->zsrc[shot])),
+              "fwi_shot_set_source");
+        check(fwi_shot_set_receivers(c, slot, geometry->nrec, rx.data(), ry.data(), rz.data()),
+              "fwi_shot_set_receivers");
+    }
+}
+
+// inversion.cu:115-170 over batches of shots.
+void Inversion::compute_gradient() {
+    sum_res = 0.0;
+    const bool final_check = iteration == max_iteration;  // last pass: misfit only
+    for (long q = 0; q < nPoints; q++) trial[q] = 1.0f / sqrtf(m[q]);
+    check(fwi_set_model_interior(ctx, trial.data()), "fwi_set_model_interior");
+    if (!final_check) {
+        check(fwi_set_model_interior(rbc, trial.data()), "fwi_set_model_interior");
+        check(fwi_zero_volume(rbc, FWI_VOL_GRAD), "fwi_zero_volume");
+        check(fwi_zero_volume(rbc, FWI_VOL_ILLUM), "fwi_zero_volume");
+    }
+    for (srcId = 0; srcId < geometry->nsrc; srcId += batch) {
+        const int count = std::min(batch, geometry->nsrc - srcId);
+        stage_info = "Computing synthetic data";
+        show_inv_info();
+        place_shots(ctx, abc_nb, srcId, count);
+        check(fwi_forward(ctx, FWI_BOUNDARY_ABC), "fwi_forward");  // get_cal_data, inversion.cu:178-188
+        for (int slot = 0; slot < count; slot++) {                  // set_adjoint_source, :190-201
+            set_obs_data(srcId + slot);
+            double s = 0.0;
+            check(fwi_shot_residual(ctx, slot, obs_data.data(), &s), "fwi_shot_residual");
+            sum_res += s;
+        }
+        if (final_check) continue;
+        stage_info = "Wavefield reconstruction: forward and backward propagation";
+        show_inv_info();
+        place_shots(rbc, rbc_nb, srcId, count);
+        for (int slot = 0; slot < batch; slot++) {  // update_RBC, :204-216
+            const int shot = srcId + (slot < count ? slot : 0);
+            check(fwi_shot_random_boundary(rbc, slot, rbc_seed, shot, rbc_amplitude, cfl_velocity()),
+                  "fwi_shot_random_boundary");
+            if (slot < count) {
+                check(fwi_shot_get_backward_data(ctx, slot, residual.data()), "fwi_shot_get_backward_data");
+            } else {
+                std::fill(residual.begin(), residual.end(), 0.0f);
+            }
+            check(fwi_shot_set_backward_data(rbc, slot, residual.data()), "fwi_shot_set_backward_data");
+        }
+        check(fwi_forward(rbc, FWI_BOUNDARY_RBC), "fwi_forward");  // :218-233
+        check(fwi_grad_backward(rbc), "fwi_grad_backward");        // :235-253
+    }
+    if (!final_check) {
+        float gmax = 0.0f;
+        check(fwi_export_gradient(rbc, zmask, gradient.data(), &gmax), "fwi_export_gradient");  // :142-169
+    }
+}
+
+// inversion.cu:280-289; the misfit is reported as sqrt(sum r^2) like the reference
+void Inversion::check_convergence() {
+    ++iteration;
+    residuo.push_back((float)std::sqrt(sum_res));
+    converged = iteration > max_iteration;
+    if (converged) std::cout << "Final residuo: " << residuo.back() << "\n\n";
+}
+
+double Inversion::model_misfit(const std::vector<float>& velocity) {
+    check(fwi_set_model_interior(ctx, velocity.data()), "fwi_set_model_interior");
+    double total = 0.0;
+    for (srcId = 0; srcId < geometry->nsrc; srcId += batch) {
+        const int count = std::min(batch, geometry->nsrc - srcId);
+        show_inv_info();
+        place_shots(ctx, abc_nb, srcId, count);
+        check(fwi_forward(ctx, FWI_BOUNDARY_ABC), "fwi_forward");
+        for (int slot = 0; slot < count; slot++) {
+            set_obs_data(srcId + slot);
+            double s = 0.0;
+            check(fwi_shot_residual(ctx, slot, obs_data.data(), &s), "fwi_shot_residual");
+            total += s;
+        }
+    }
+    return total;
+}
+
+// inversion.cu:315-347
+double Inversion::linesearch(float alpha) {
+    for (long q = 0; q < nPoints; q++) {
+        trial[q] = m[q] - alpha * gradient[q];
+        trial[q] = 1.0f / sqrtf(trial[q]);
+    }
+    sum_res = model_misfit(trial);
+    return sum_res;
+}
+
+// inversion.cu:291-312: parabola through (0, f0), (a1, f1), (a2, f2).
+void Inversion::optimization() {
+    const float dm = (1.0f / (vmin * vmin)) - (1.0f / (vmax * vmax));
+    const float a0 = 0.00f * dm, a1 = 0.02f * dm, a2 = 0.05f * dm;
+    const float f0 = residuo.back();
+    stage_info = "Optimization via parabolic linesearch: first modeling";
+    const float f1 = (float)std::sqrt(linesearch(a1));
+    stage_info = "Optimization via parabolic linesearch: final modeling";
+    const float f2 = (float)std::sqrt(linesearch(a2));
+    const float num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2;
+    const float den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2;
+    step = 0.5f * (num / den);
+}
+
+// inversion.cu:349-362
+void Inversion::update_model() {
+    for (long q = 0; q < nPoints; q++) {
+        m[q] = m[q] - step * gradient[q];
+        trial[q] = 1.0f / sqrtf(m[q]);
+    }
+    const std::string file = output_folder + "model_FWI_iteration_" + std::to_string(iteration) + "_" +
+                             std::to_string(int(fmax)) + "Hz_" + std::to_string(nz) + "x" + std::to_string(nx) + "x" +
+                             std::to_string(ny) + "_" + std::to_string((int)(dh)) + "m.bin";
+    export_binary_float(file, trial.data(), nPoints);
+}
+
+// inversion.cu:364-376
+void Inversion::export_convergence() {
+    const std::string path = residuo_folder + "convergence_" + std::to_string(max_iteration) + "_iterations.txt";
+    std::ofstream resFile(path, std::ios::out);
+    for (int r = 0; r <= max_iteration && r < (int)residuo.size(); r++) resFile << residuo[r] << "\n";
+    std::cout << "Text file " << path << " was successfully written." << std::endl;
+}
+
+// inversion.cu:378-383
+void Inversion::export_final_model() {
+    for (long q = 0; q < nPoints; q++) trial[q] = 1.0f / sqrtf(m[q]);
+    const std::string file = output_folder + "final_model_FWI_" + std::to_string(int(fmax)) + "Hz_" +
+                             std::to_string(nz) + "x" + std::to_string(nx) + "x" + std::to_string(ny) + "_" +
+                             std::to_string((int)(dh)) + "m.bin";
+    export_binary_float(file, trial.data(), nPoints);
+}
+
+// inversion.cu:255-278 (without clearing the terminal)
+void Inversion::show_inv_info() {
+    show_information();
+    const std::string line(80, '-');
+    std::cout << line << "\n" << stage_info << "\n" << line << "\n\n";
+    if (iteration > max_iteration)
+        std::cout << "-------- Checking final residuo --------\n\n";
+    else if (iteration == 0)
+        std::cout << "-------- Computing first residuo --------\n";
+    else
+        std::cout << "-------- Computing iteration " << iteration << " of " << max_iteration
+                  << " --------\n\nPrevious residuo: " << residuo.back() << "\n\n";
+}

@@ -32,7 +32,9 @@ EXPORTS = [
     "fwi_grad_backward", "fwi_backward_steps", "fwi_zero_volume", "fwi_get_volume", "fwi_set_volume",
     "fwi_export_image", "fwi_export_gradient", "fwi_residual", "fwi_autotune", "fwi_get_tuning",
     "fwi_set_tuning", "fwi_set_graph", "fwi_last_pass_timing", "fwi_set_kernel_timing",
-    "fwi_last_kernel_timing", "fwi_sync",
+    "fwi_last_kernel_timing", "fwi_sync", "fwi_create_batch", "fwi_shot_random_boundary",
+    "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
+    "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
 ]
 
 
@@ -94,6 +96,17 @@ def load():
         "fwi_set_kernel_timing": [P, I],
         "fwi_last_kernel_timing": [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I)],
         "fwi_sync": [P],
+        "fwi_create_batch": [ctypes.POINTER(Grid), I, I, ctypes.POINTER(P)],
+        "fwi_shot_random_boundary": [P, I, ctypes.c_uint64, I, F, F],
+        "fwi_shot_set_source": [P, I, I, I, I],
+        "fwi_shot_set_receivers": [P, I, I, i32, i32, i32],
+        "fwi_shot_get_seismogram": [P, I, f32],
+        "fwi_shot_set_backward_data": [P, I, f32],
+        "fwi_shot_residual": [P, I, f32, ctypes.POINTER(ctypes.c_double)],
+        "fwi_shot_get_volume": [P, I, I, f32],
+        "fwi_set_shots_per_block": [P, I, I],
+        "fwi_get_shots_per_block": [P, I, ctypes.POINTER(I)],
+        "fwi_shot_get_backward_data": [P, I, f32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -109,14 +122,15 @@ class Context:
     """One padded grid on one GPU: the state the reference keeps in its
     Modeling / Migration / Inversion objects (modeling.cuh:18-90)."""
 
-    def __init__(self, nxx, nyy, nzz, nb, dh, dt, nt, tlag, device=0):
+    def __init__(self, nxx, nyy, nzz, nb, dh, dt, nt, tlag, device=0, nshots=1):
         self.L = load()
         self.grid = Grid(nxx, nyy, nzz, nb, dh, dt, nt, tlag)
         self.shape = (nyy, nxx, nzz)
         self.nb, self.nt, self.tlag, self.dh, self.dt = nb, nt, tlag, dh, dt
-        self.nrec = 0
+        self.nshots = nshots
+        self.nrec_slot = [0] * nshots
         h = ctypes.c_void_p()
-        rc = self.L.fwi_create(ctypes.byref(self.grid), device, ctypes.byref(h))
+        rc = self.L.fwi_create_batch(ctypes.byref(self.grid), nshots, device, ctypes.byref(h))
         if rc != OK:
             raise FwiError(rc, self.L.fwi_last_error(None).decode())
         self.h = h
@@ -179,14 +193,29 @@ class Context:
             raise ValueError("b1d needs nb samples")
         self._chk(self.L.fwi_set_dampers(self.h, b1d))
 
-    def set_source(self, sx, sy, sz):
-        self._chk(self.L.fwi_set_source(self.h, int(sx), int(sy), int(sz)))
+    @property
+    def nrec(self):
+        return self.nrec_slot[0]
 
-    def set_receivers(self, rec):
+    def set_source(self, sx, sy, sz, slot=0):
+        self._chk(self.L.fwi_shot_set_source(self.h, slot, int(sx), int(sy), int(sz)))
+
+    def set_receivers(self, rec, slot=0):
         rec = np.asarray(rec, np.int32).reshape(-1, 3)
         rx, ry, rz = (np.ascontiguousarray(rec[:, q]) for q in range(3))
-        self._chk(self.L.fwi_set_receivers(self.h, len(rec), rx, ry, rz))
-        self.nrec = len(rec)
+        self._chk(self.L.fwi_shot_set_receivers(self.h, slot, len(rec), rx, ry, rz))
+        self.nrec_slot[slot] = len(rec)
+
+    def shot_random_boundary(self, slot, seed, shot, amp, vclamp):
+        self._chk(self.L.fwi_shot_random_boundary(self.h, slot, seed, shot, amp, vclamp))
+
+    def set_shots_per_block(self, kind, sb):
+        self._chk(self.L.fwi_set_shots_per_block(self.h, kind, sb))
+
+    def shots_per_block(self, kind):
+        sb = ctypes.c_int()
+        self._chk(self.L.fwi_get_shots_per_block(self.h, kind, ctypes.byref(sb)))
+        return sb.value
 
     # -- passes -------------------------------------------------------------
     def forward(self, boundary=BOUNDARY_RBC):
@@ -195,16 +224,21 @@ class Context:
     def forward_steps(self, boundary, n0, n1):
         self._chk(self.L.fwi_forward_steps(self.h, boundary, n0, n1))
 
-    def seismogram(self):
-        out = np.zeros((self.nrec, self.nt), np.float32)
-        self._chk(self.L.fwi_get_seismogram(self.h, out))
+    def seismogram(self, slot=0):
+        out = np.zeros((self.nrec_slot[slot], self.nt), np.float32)
+        self._chk(self.L.fwi_shot_get_seismogram(self.h, slot, out))
         return out
 
-    def set_backward_data(self, data):
+    def set_backward_data(self, data, slot=0):
         data = np.ascontiguousarray(data, np.float32)
-        if data.shape != (self.nrec, self.nt):
+        if data.shape != (self.nrec_slot[slot], self.nt):
             raise ValueError("data must be (nrec, nt)")
-        self._chk(self.L.fwi_set_backward_data(self.h, data))
+        self._chk(self.L.fwi_shot_set_backward_data(self.h, slot, data))
+
+    def backward_data(self, slot=0):
+        out = np.zeros((self.nrec_slot[slot], self.nt), np.float32)
+        self._chk(self.L.fwi_shot_get_backward_data(self.h, slot, out))
+        return out
 
     def rtm_backward(self):
         self._chk(self.L.fwi_rtm_backward(self.h))
@@ -216,9 +250,9 @@ class Context:
         self._chk(self.L.fwi_backward_steps(self.h, kind, b0, b1))
 
     # -- volumes -------------------------------------------------------------
-    def get_volume(self, which):
+    def get_volume(self, which, slot=0):
         out = np.zeros(self.shape, np.float32)
-        self._chk(self.L.fwi_get_volume(self.h, which, out))
+        self._chk(self.L.fwi_shot_get_volume(self.h, slot, which, out))
         return out
 
     def set_volume(self, which, vol):
@@ -242,12 +276,12 @@ class Context:
         self._chk(self.L.fwi_export_gradient(self.h, zmask, out, ctypes.byref(g)))
         return out, g.value
 
-    def residual(self, obs):
+    def residual(self, obs, slot=0):
         obs = np.ascontiguousarray(obs, np.float32)
-        if obs.shape != (self.nrec, self.nt):
+        if obs.shape != (self.nrec_slot[slot], self.nt):
             raise ValueError("obs must be (nrec, nt)")
         s = ctypes.c_double(0.0)
-        self._chk(self.L.fwi_residual(self.h, obs, ctypes.byref(s)))
+        self._chk(self.L.fwi_shot_residual(self.h, slot, obs, ctypes.byref(s)))
         return s.value
 
     # -- tuning / timing ------------------------------------------------------

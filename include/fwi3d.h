@@ -166,9 +166,38 @@ int fwi_export_gradient(fwi_ctx* ctx, float zmask, float* out_interior, float* g
  * in sequential FP32 on the host). */
 int fwi_residual(fwi_ctx* ctx, const float* obs, double* sum_sq);
 
-/* Autotune tile shape and z-chunking for a pass kind (0 forward RBC,
- * 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing `steps` steps of
- * each candidate; cache_path (may be NULL) persists the choice. */
+/* ---- Batched shots -------------------------------------------------------
+ * A batch context holds `nshots` shot slots that advance together: one step
+ * kernel per time step updates every slot (shots of uneven receiver aperture
+ * included), and the backward pass sums all slots into one image /
+ * illumination / gradient.  The reference runs shots one after the other
+ * (modeling_main.cpp:13-22, migration_main.cpp:13-19,
+ * inversion.cu:123-140); a batch replaces that loop.  Models set with
+ * fwi_set_model / fwi_set_model_interior go to every slot; the calls below
+ * address one slot.  Entry points without a slot argument address slot 0. */
+int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out);
+int fwi_shot_random_boundary(fwi_ctx* ctx, int slot, uint64_t seed, int shot, float amp, float vclamp);
+int fwi_shot_set_source(fwi_ctx* ctx, int slot, int sIdx, int sIdy, int sIdz);
+int fwi_shot_set_receivers(fwi_ctx* ctx, int slot, int nrec, const int* rIdx, const int* rIdy, const int* rIdz);
+int fwi_shot_get_seismogram(fwi_ctx* ctx, int slot, float* out);
+int fwi_shot_set_backward_data(fwi_ctx* ctx, int slot, const float* data);
+int fwi_shot_residual(fwi_ctx* ctx, int slot, const float* obs, double* sum_sq);
+/* The slot's backward data (after fwi_shot_residual: the residual), nt*nrec
+ * floats, t + r*nt -- the seismogram the reference copies back to the host
+ * in set_adjoint_source (inversion.cu:192-200). */
+int fwi_shot_get_backward_data(fwi_ctx* ctx, int slot, float* out);
+int fwi_shot_get_volume(fwi_ctx* ctx, int slot, int which, float* out_padded);
+
+/* Shots advanced by one block (1, 2 or 4; must divide nshots): the shots of
+ * a block share the image / illumination read-modify-write. */
+int fwi_set_shots_per_block(fwi_ctx* ctx, int kind, int sb);
+int fwi_get_shots_per_block(fwi_ctx* ctx, int kind, int* sb);
+
+/* Autotune tile shape, z-chunking and shots per block for a pass kind
+ * (0 forward RBC, 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing
+ * `steps` steps of each candidate; cache_path (may be NULL) persists the
+ * choice.  The trial steps run on the context's buffers, which are zeroed
+ * afterwards: autotune before setting up a pass. */
 int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);
 
 /* Current launch configuration: tile rows per block, rows per thread,

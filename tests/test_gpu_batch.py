@@ -1,0 +1,141 @@
+"""Batched multi-shot launches against the per-shot path (run with -m gpu).
+
+Shots of uneven aperture (receiver patches clipped at the model edges) share
+one step kernel per time step.  Forward seismograms are per-shot independent
+and must match single-shot runs bit for bit; the stacked RTM image sums the
+shots in a different order than sequential accumulation and is held to
+relative L2 <= 1e-4 (the bound the batched RTM target states)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.cases import Case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_BATCH = 1e-4
+
+
+@pytest.fixture(scope="module")
+def fl():
+    from fwi3d_amd import lib
+    lib.load()
+    return lib
+
+
+def _shots(case, nshots):
+    """Sources on a grid near the surface, each with a receiver patch of
+    half-width 10 cells centred on it and clipped to the interior."""
+    nb = case.nb
+    xs = np.linspace(nb + 4, nb + case.nx - 5, 2 if nshots > 1 else 1).astype(int)
+    ys = np.linspace(nb + 4, nb + case.ny - 5, max(1, nshots // 2)).astype(int)
+    out = []
+    for sy in ys:
+        for sx in xs:
+            gx = np.arange(sx - 10, sx + 11, 2)
+            gy = np.arange(sy - 10, sy + 11, 2)
+            gx = gx[(gx >= nb) & (gx < nb + case.nx)]
+            gy = gy[(gy >= nb) & (gy < nb + case.ny)]
+            X, Y = np.meshgrid(gx, gy, indexing="ij")
+            rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, nb + 2)], axis=1).astype(np.int32)
+            out.append(((int(sx), int(sy), nb + 2), rec))
+    return out[:nshots]
+
+
+def _single(fl, case, interior, shot, src, rec, data=None, image=None, illum=None):
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag)
+    c.set_model_interior(interior)
+    c.random_boundary(99, shot, 0.6, float(case.vclamp))
+    c.set_wavelet(case.wavelet)
+    c.set_source(*src)
+    c.set_receivers(rec)
+    if image is not None:
+        c.set_volume(fl.VOL_IMAGE, image)
+        c.set_volume(fl.VOL_ILLUM, illum)
+    c.forward(fl.BOUNDARY_RBC)
+    seis = c.seismogram()
+    if data is not None:
+        c.set_backward_data(data)
+        c.rtm_backward()
+        image, illum = c.get_volume(fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)
+    c.close()
+    return seis, image, illum
+
+
+@pytest.mark.parametrize("sb", [1, 2, 4])
+def test_batched_rtm_matches_per_shot(fl, sb):
+    case = Case(nx=48, ny=44, nz=32, nt=70, amp=0.0)
+    S = 4
+    shots = _shots(case, S)
+    # per-shot reference path: sequential shots accumulating into one image
+    seis_ref, image, illum = [], None, None
+    for s, (src, rec) in enumerate(shots):
+        seis, _, _ = _single(fl, case, case.interior, s, src, rec)
+        seis_ref.append(seis)
+    zero = np.zeros((case.nyy, case.nxx, case.nzz), np.float32)
+    image, illum = zero, zero
+    for s, (src, rec) in enumerate(shots):
+        _, image, illum = _single(fl, case, case.interior, s, src, rec, data=seis_ref[s], image=image, illum=illum)
+
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    for s, (src, rec) in enumerate(shots):
+        c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+        c.set_source(*src, slot=s)
+        c.set_receivers(rec, slot=s)
+    c.set_wavelet(case.wavelet)
+    for kind in (0, 2):
+        c.set_shots_per_block(kind, sb)
+    c.forward(fl.BOUNDARY_RBC)
+    for s in range(S):
+        assert np.array_equal(c.seismogram(slot=s), seis_ref[s]), s
+        c.set_backward_data(seis_ref[s], slot=s)
+    c.rtm_backward()
+    img_b = c.get_volume(fl.VOL_IMAGE)
+    ill_b = c.get_volume(fl.VOL_ILLUM)
+    c.close()
+    assert len({len(r) for _, r in shots}) > 1 or S == 1  # uneven apertures exercised
+    assert np.abs(image).max() > 0
+    assert rel_l2(img_b, image) <= TOL_BATCH, rel_l2(img_b, image)
+    assert rel_l2(ill_b, illum) <= TOL_BATCH, rel_l2(ill_b, illum)
+
+
+def test_batched_models_match_oracle(fl):
+    case = Case(amp=0.0)
+    S = 2
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    for s in range(S):
+        c.shot_random_boundary(s, 5, 10 + s, 0.6, float(case.vclamp))
+    pad = O.expand_boundary(case.interior, case.nb)
+    for s in range(S):
+        want = O.random_boundary(pad, case.nb, 5, 10 + s, 0.6, case.vclamp)
+        assert np.array_equal(c.get_volume(fl.VOL_MODEL, slot=s), want)
+    c.close()
+
+
+def test_batched_residual_and_fwi_gradient(fl):
+    case = Case(nx=40, ny=40, nz=28, nt=50, amp=0.0, abc_factor=0.05)
+    S = 2
+    shots = _shots(case, S)
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    c.set_wavelet(case.wavelet)
+    c.set_dampers(case.abc[0])
+    for s, (src, rec) in enumerate(shots):
+        c.set_source(*src, slot=s)
+        c.set_receivers(rec, slot=s)
+    c.forward(fl.BOUNDARY_ABC)
+    sums = []
+    for s, (src, rec) in enumerate(shots):
+        cal = c.seismogram(slot=s)
+        vp = O.expand_boundary(case.interior, case.nb)
+        _, _, want = O.forward(vp, case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, src, rec,
+                               abc_tables=case.abc)
+        assert np.array_equal(cal, want)
+        obs = (cal * 0.9).astype(np.float32)
+        got = c.residual(obs, slot=s)
+        _, ref, _ = O.misfit(obs, cal)
+        assert abs(got - ref) <= 1e-12 * ref
+        sums.append(got)
+    c.close()

@@ -1,0 +1,89 @@
+"""End-to-end runs of the modeling / migration / inversion drivers on a tiny
+parameters.txt (run with -m gpu)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "bin")
+
+
+def _run(exe, params):
+    r = subprocess.run([os.path.join(BIN, exe), params], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "Run time" in r.stdout
+    return r.stdout
+
+
+def _seis(folder, w, shot):
+    f = os.path.join(folder, "inputs/data", "seismogram_nt%d_nr%d_%dus_shot_%d.bin"
+                     % (w.nt, len(w.receivers[0]), int(np.float32(1e6) * np.float32(w.dt)), shot + 1))
+    return np.fromfile(f, np.float32).reshape(len(w.receivers[0]), w.nt)
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    from fwi3d_amd import configs
+    return configs.tiny(nshots=3)
+
+
+def test_modeling_main_matches_oracle(tiny, tmp_path):
+    w = tiny
+    folder = str(tmp_path / "run")
+    _run("modeling_main", w.write(folder))
+    vp = O.expand_boundary(w.interior, w.nb)
+    tables = O.dampers(w.nb, 0.004)
+    for s in range(len(w.shots)):
+        _, _, want = O.forward(vp, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src_index(s), w.rec_index(s),
+                               abc_tables=tables)
+        assert np.array_equal(_seis(folder, w, s), want), s
+
+
+def test_modeling_main_batched_is_identical(tiny, tmp_path):
+    w = tiny
+    a, b = str(tmp_path / "seq"), str(tmp_path / "bat")
+    _run("modeling_main", w.write(a))
+    _run("modeling_main", w.write(b, batch_shots=2))  # 3 shots: one full batch + one partial
+    for s in range(len(w.shots)):
+        assert np.array_equal(_seis(a, w, s), _seis(b, w, s)), s
+
+
+def test_migration_main_sequential_vs_batched(tiny, tmp_path):
+    w = tiny
+    imgs = []
+    for name, batch in (("seq", 1), ("bat", 3)):
+        folder = str(tmp_path / name)
+        params = w.write(folder, batch_shots=batch)
+        _run("modeling_main", params)
+        _run("migration_main", params)
+        f = os.path.join(folder, "outputs/seismic", "RTM_section_45Hz_%dx%dx%d_10m.bin" % (w.nz, w.nx, w.ny))
+        img = np.fromfile(f, np.float32).reshape(w.ny, w.nx, w.nz)
+        ill = np.fromfile(f.replace("RTM_section", "RTM_illumination"), np.float32)
+        assert np.isfinite(img).all() and np.abs(img).max() > 0 and ill.min() > 0
+        imgs.append(img)
+    assert rel_l2(imgs[1], imgs[0]) <= 1e-4
+
+
+def test_inversion_main_lowers_misfit(tiny, tmp_path):
+    from scipy.ndimage import gaussian_filter
+    w = tiny
+    folder = str(tmp_path / "fwi")
+    _run("modeling_main", w.write(folder))  # observed data from the true model
+    init = (1.0 / gaussian_filter(1.0 / w.interior.astype(np.float64), 2.0)).astype(np.float32)
+    params = w.write(folder, model=init, kind="init", max_iteration=2, batch_shots=3, depth_mask=20.0)
+    out = _run("inversion_main", params)
+    conv = np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_2_iterations.txt"))
+    assert conv.shape == (3,), out[-1500:]
+    assert conv[-1] < conv[0], conv
+    final = np.fromfile(os.path.join(folder, "outputs/models", "final_model_FWI_45Hz_%dx%dx%d_10m.bin"
+                                     % (w.nz, w.nx, w.ny)), np.float32).reshape(w.interior.shape)
+    assert np.isfinite(final).all()
+    assert rel_l2(final, w.interior) < rel_l2(init, w.interior)
