@@ -219,6 +219,30 @@ int fwi_last_pass_timing(fwi_ctx* ctx, double* ms, int* launches);
 int fwi_set_kernel_timing(fwi_ctx* ctx, int enable);
 int fwi_last_kernel_timing(fwi_ctx* ctx, double* ms, int* launches);
 
+/* ---- Learned preconditioner ------------------------------------------------
+ * No counterpart in the reference.  A 3-level 3-D U-Net (16/32/64 channels,
+ * 3x3x3 convolutions, BF16 weights and activations, FP32 accumulation on the
+ * matrix cores) maps (gradient, illumination, model) to a model update.
+ * Inputs x3 are planar FP32 [3][B][D][H][W] (D, H, W divisible by 4; x
+ * fastest); outputs / targets are FP32 [B][D][H][W]. */
+typedef struct fwi_unet fwi_unet;
+int fwi_unet_create(int device, uint64_t seed, fwi_unet** out);
+int fwi_unet_destroy(fwi_unet* net);
+const char* fwi_unet_last_error(const fwi_unet* net);
+long fwi_unet_num_params(const fwi_unet* net);
+int fwi_unet_get_params(fwi_unet* net, float* out);
+int fwi_unet_set_params(fwi_unet* net, const float* in);
+int fwi_unet_forward(fwi_unet* net, const float* x3, int B, int D, int H, int W, float* y);
+/* MSE loss mean((y - target)^2) and the gradient of every parameter. */
+int fwi_unet_gradients(fwi_unet* net, const float* x3, const float* target, int B, int D, int H, int W,
+                       double* loss, float* grad);
+/* One Adam step (beta 0.9 / 0.999) on the MSE loss. */
+int fwi_unet_train_step(fwi_unet* net, const float* x3, const float* target, int B, int D, int H, int W, float lr,
+                        double* loss);
+/* One convolution layer alone (FP32 output, no bias/activation), for tests;
+ * in is channels-last FP32 [B][D][H][W][Cin] rounded to BF16 on upload. */
+int fwi_unet_conv_probe(fwi_unet* net, int layer, const float* in, int B, int D, int H, int W, float* out);
+
 /* Synchronise the context's stream. */
 int fwi_sync(fwi_ctx* ctx);
 

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fwi[a-z0-9_]*)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long|const char\*)\s+(fwi[a-z0-9_]*)\s*\(", text, re.M)))
 
 
 def test_hip_library_exports_header():
@@ -21,7 +21,8 @@ def test_hip_library_exports_header():
     L = lib.load()
     missing = [s for s in declared if not hasattr(L, s)]
     assert not missing, missing
-    assert sorted(lib.EXPORTS) == sorted(declared)
+    from fwi3d_amd import unet
+    assert sorted(lib.EXPORTS + unet.EXPORTS) == sorted(declared)
 
 
 def test_create_without_device_fails_cleanly():
