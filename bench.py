@@ -62,6 +62,75 @@ def cpu_baseline(w, vp_pad, seconds):
                       % (nf, nbk, t_fwd + t_bwd)}
 
 
+def batched_rtm(w, single, nshots, lib, configs):
+    """RTM of the first `nshots` shots of the 4x4 surface survey (each shot its
+    own random boundary and its own 201x201 receiver patch clipped at the
+    model edges), once as one batch and once shot after shot on the
+    single-shot context; returns shots/h of both and the image difference."""
+    w16 = configs.gradient_401_16(interior=w.interior)
+    S = nshots
+    log("batched RTM: %d shots, receivers per shot %s" % (S, [len(w16.receivers[s]) for s in range(S)]))
+    c = lib.Context(w.nxx, w.nyy, w.nzz, w.nb, w.dh, w.dt, w.nt, w.tlag, nshots=S)
+    c.set_model_interior(w.interior)
+    c.set_wavelet(w.wavelet)
+    for s in range(S):
+        c.shot_random_boundary(s, w.rbc_seed, s, w.rbc_amp, w.vclamp)
+        c.set_source(*w16.src_index(s), slot=s)
+        c.set_receivers(w16.rec_index(s), slot=s)
+    t0 = time.perf_counter()
+    c.autotune(lib.KIND_FWD_RBC, 3)
+    c.autotune(lib.KIND_BWD_RTM, 3)
+    log("batched autotune %.1f s: forward %s sb=%d, rtm %s sb=%d" % (
+        time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.tuning(2), c.shots_per_block(2)))
+    c.forward(lib.BOUNDARY_RBC)
+    data = [c.seismogram(slot=s) for s in range(S)]
+    for s in range(S):
+        c.set_backward_data(data[s], slot=s)
+    c.zero_volume(lib.VOL_IMAGE)
+    c.zero_volume(lib.VOL_ILLUM)
+    c.sync()
+    t0 = time.perf_counter()
+    c.forward(lib.BOUNDARY_RBC)
+    f_ms = c.last_pass_timing()[0]
+    c.rtm_backward()
+    b_ms = c.last_pass_timing()[0]
+    c.sync()
+    t_batch = time.perf_counter() - t0
+    img_b = c.get_volume(lib.VOL_IMAGE)
+    c.close()
+    log("batched: %.2f s (forward %.0f ms, backward %.0f ms)" % (t_batch, f_ms, b_ms))
+
+    # per-shot path on the single-shot context
+    single.zero_volume(lib.VOL_IMAGE)
+    single.zero_volume(lib.VOL_ILLUM)
+    single.sync()
+    t_seq = 0.0
+    for s in range(S):
+        # set-up (model, boundary, geometry, data upload) is outside the timing
+        single.set_model_interior(w.interior)
+        single.random_boundary(w.rbc_seed, s, w.rbc_amp, w.vclamp)
+        single.set_source(*w16.src_index(s))
+        single.set_receivers(w16.rec_index(s))
+        single.set_backward_data(data[s])
+        single.sync()
+        t0 = time.perf_counter()
+        single.forward(lib.BOUNDARY_RBC)
+        single.rtm_backward()
+        single.sync()
+        t_seq += time.perf_counter() - t0
+    img_s = single.get_volume(lib.VOL_IMAGE)
+    rel = float(np.linalg.norm((img_b - img_s).astype(np.float64)) / np.linalg.norm(img_s.astype(np.float64)))
+    log("per-shot: %.2f s; image rel-L2 batched vs per-shot %.3g" % (t_seq, rel))
+    cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
+    return {"shots": S, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
+            "shots_per_hour_per_shot": round(3600.0 * S / t_seq, 1),
+            "speedup": round(t_seq / t_batch, 3),
+            "gcell_upd_per_s_batched": round(cells * w.steps * 3 * S / t_batch / 1e9, 2),
+            "image_rel_l2_vs_per_shot": rel,
+            "note": "wall time of forward + RTM backward passes only; model, boundary, geometry and data "
+                    "set-up are outside the timing on both paths"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +144,12 @@ def main():
     ap.add_argument("--pass-steps", type=int, default=0,
                     help="profiling only: cut each pass to this many time steps")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="profiling only: also run a plain 1 GiB device copy (counter calibration)")
+                    help="profiling only: one 1 GiB copy-kernel launch (counter calibration)")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch shapes instead of autotuning, e.g. 'fwd=4,2,8;rtm=16,1,1'")
+    ap.add_argument("--batch-shots", type=int, default=4,
+                    help="also time this many shots of the 4x4 surface survey as one batch against the "
+                         "per-shot path (0: skip; 16: the full survey)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,7 +171,13 @@ def main():
     ctx.set_wavelet(w.wavelet)
     ctx.set_source(*w.src)
     ctx.set_receivers(w.rec)
-    if not args.no_autotune:
+    if args.tuning:
+        kinds = {"fwd": lib.KIND_FWD_RBC, "rtm": lib.KIND_BWD_RTM}
+        for item in args.tuning.split(";"):
+            name, vals = item.split("=")
+            ctx.set_tuning(kinds[name.strip()], *[int(v) for v in vals.split(",")])
+        log("fixed tuning: forward %s, rtm %s" % (ctx.tuning(0), ctx.tuning(2)))
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         ctx.autotune(lib.KIND_FWD_RBC, 4)
         ctx.autotune(lib.KIND_BWD_RTM, 4)
@@ -105,9 +185,13 @@ def main():
     ctx.set_graph(args.graph)
     ctx.set_kernel_timing(not args.graph)
 
-    # synthetic observed data: the forward pass's own recording
-    ctx.forward(lib.BOUNDARY_RBC)
-    ctx.set_backward_data(ctx.seismogram())
+    if args.pass_steps:
+        # profiling: the injected data's values do not change the traffic
+        ctx.set_backward_data(np.zeros((ctx.nrec, w.nt), np.float32))
+    else:
+        # synthetic observed data: the forward pass's own recording
+        ctx.forward(lib.BOUNDARY_RBC)
+        ctx.set_backward_data(ctx.seismogram())
     log("data ready")
 
     def step():
@@ -123,20 +207,8 @@ def main():
         b = ctx.last_pass_timing() + ctx.last_kernel_timing()
         return f, b
 
-    if args.copy_probe:
-        import torch
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / 10 * 1e3
-        log("copy probe: 1 GiB copy %.3f ms -> %.1f GB/s (read+write)" % (ms, 2 * (1 << 30) / (ms * 1e-3) / 1e9))
-        del src, dst
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 1 if args.copy_probe else 10)
+    log("copy kernel: %.1f GB/s (read + write)" % copy_gbs)
 
     for i in range(args.warmup):
         step()
@@ -175,12 +247,17 @@ def main():
         ach = bwd_bytes / bwd_avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac_of_copy_kernel": round(ach / copy_gbs, 4),
                     "kernel": "step_kernel<BWD_RTM> (fused reconstruction + receiver propagation + imaging)",
                     "avg_launch_us": round(bwd_avg_s * 1e6, 2), "algorithmic_bytes_per_launch": bwd_bytes}
         extra["forward_kernel"] = {"avg_launch_us": round(fwd_avg_s * 1e6, 2),
                                    "achieved_GBs": round(fwd_bytes / fwd_avg_s / 1e9, 1),
                                    "frac": round(fwd_bytes / fwd_avg_s / 1e9 / HBM_PEAK_GBS, 4)}
     pct_hbm = value * BYTES_PER_UPDATE / HBM_PEAK_GBS
+
+    batched = None
+    if args.batch_shots > 0 and not args.pass_steps:
+        batched = batched_rtm(w, ctx, args.batch_shots, lib, configs)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -218,6 +295,8 @@ def main():
         "pct_hbm_at_16B_per_update": round(pct_hbm, 4),
         "forward_pass_ms": round(fwd_pass / args.steps, 2),
         "rtm_backward_pass_ms": round(bwd_pass / args.steps, 2),
+        "batched_rtm": batched,
+        "copy_kernel_GBs": round(copy_gbs, 1),
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

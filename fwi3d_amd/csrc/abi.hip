@@ -866,8 +866,15 @@ int fwi_set_shots_per_block(fwi_ctx* c, int kind, int sb) {
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
     Tuning t = c->tune[kind];
     t.sb = sb;
-    if (sb < 1 || c->nshots % sb != 0 || !fwi::step_config_supported(kind, c->arith, t))
-        return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
+    if (sb < 1 || c->nshots % sb != 0) return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
+    // keep the current tile if it exists for this sb, else take the first one that does
+    if (!fwi::step_config_supported(kind, c->arith, t))
+        for (const auto& tc : kTileChoices) {
+            t.by = tc[0];
+            t.rpt = tc[1];
+            if (fwi::step_config_supported(kind, c->arith, t)) break;
+        }
+    if (!fwi::step_config_supported(kind, c->arith, t)) return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
     c->tune[kind] = t;
     return FWI_OK;
 }
@@ -982,6 +989,28 @@ int fwi_last_kernel_timing(fwi_ctx* c, double* ms, int* launches) {
     if (!c || !ms || !launches) return FWI_ERR_ARG;
     *ms = c->kernel_ms;
     *launches = c->kev_used;
+    return FWI_OK;
+}
+
+int fwi_copy_bandwidth(fwi_ctx* c, long bytes, int reps, double* gbps) {
+    if (!c || !gbps || bytes < 1024 || reps < 1) return c ? fail(c, FWI_ERR_ARG, "bytes/reps") : FWI_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const long n = bytes / 16 * 4;
+    float *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc((void**)&a, n * sizeof(float)));
+    hipError_t e = hipMalloc((void**)&b, n * sizeof(float));
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * sizeof(float), c->st);
+    if (e == hipSuccess) e = fwi::launch_copy(a, b, n, c->st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = fwi::launch_copy(a, b, n, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(c, FWI_ERR_HIP, std::string("copy bandwidth: ") + hipGetErrorString(e));
+    *gbps = 2.0 * n * sizeof(float) * reps / (ms * 1e-3) / 1e9;
     return FWI_OK;
 }
 

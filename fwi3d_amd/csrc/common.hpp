@@ -101,6 +101,7 @@ hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t 
                                   float vclamp, hipStream_t st);
 hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, float value, hipStream_t st);
 hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st);
+hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st);
 hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec,
                          int ncells, const float* data_row, float idh3, hipStream_t st);
 hipError_t launch_export_image(const float* image, const float* illum, float* out, const Layout& L, int nb,

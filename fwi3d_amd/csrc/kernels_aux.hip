@@ -315,6 +315,17 @@ hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, f
     return hipGetLastError();
 }
 
+// Plain streaming copy (16-byte vectors): the calibration kernel whose
+// bandwidth and HBM counters the stencil kernels are compared with.
+__global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, long n) {
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) dst[q] = src[q];
+}
+
+hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st) {
+    copy_kernel<<<8192, 256, 0, st>>>((const float4*)src, (float4*)dst, nfloats / 4);
+    return hipGetLastError();
+}
+
 hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st) {
     if (groups <= 1) return hipSuccess;
     fold_groups_kernel<<<4096, 256, 0, st>>>(v, vol, groups);

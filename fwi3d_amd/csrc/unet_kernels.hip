@@ -418,6 +418,8 @@ hipError_t channel_slice(const float* src, int Cs, int c0, float* dst, int Cd, l
                                                                    B, D, H, W); break;                             \
         case 2: conv_fwd_kernel<TA, 2, OUT><<<grid, 256, 0, st>>>(inA, Ca, inB, Cb, wt, Kpad, bias, Cout, relu, out, \
                                                                    B, D, H, W); break;                             \
+        case 3: conv_fwd_kernel<TA, 3, OUT><<<grid, 256, 0, st>>>(inA, Ca, inB, Cb, wt, Kpad, bias, Cout, relu, out, \
+                                                                   B, D, H, W); break;                             \
         case 4: conv_fwd_kernel<TA, 4, OUT><<<grid, 256, 0, st>>>(inA, Ca, inB, Cb, wt, Kpad, bias, Cout, relu, out, \
                                                                    B, D, H, W); break;                             \
         case 6: conv_fwd_kernel<TA, 6, OUT><<<grid, 256, 0, st>>>(inA, Ca, inB, Cb, wt, Kpad, bias, Cout, relu, out, \

@@ -35,6 +35,7 @@ EXPORTS = [
     "fwi_last_kernel_timing", "fwi_sync", "fwi_create_batch", "fwi_shot_random_boundary",
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
+    "fwi_copy_bandwidth",
 ]
 
 
@@ -107,6 +108,7 @@ def load():
         "fwi_set_shots_per_block": [P, I, I],
         "fwi_get_shots_per_block": [P, I, ctypes.POINTER(I)],
         "fwi_shot_get_backward_data": [P, I, f32],
+        "fwi_copy_bandwidth": [P, ctypes.c_long, I, ctypes.POINTER(ctypes.c_double)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -311,6 +313,11 @@ class Context:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._chk(self.L.fwi_last_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        g = ctypes.c_double()
+        self._chk(self.L.fwi_copy_bandwidth(self.h, nbytes, reps, ctypes.byref(g)))
+        return g.value
 
     def sync(self):
         self._chk(self.L.fwi_sync(self.h))

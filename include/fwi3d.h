@@ -243,6 +243,11 @@ int fwi_unet_train_step(fwi_unet* net, const float* x3, const float* target, int
  * in is channels-last FP32 [B][D][H][W][Cin] rounded to BF16 on upload. */
 int fwi_unet_conv_probe(fwi_unet* net, int layer, const float* in, int B, int D, int H, int W, float* out);
 
+/* Device-to-device copy bandwidth [GB/s, read + write] of a plain streaming
+ * copy kernel over `bytes`, `reps` times: the measured reference the stencil
+ * kernels' bandwidth is compared with.  No counterpart in the reference. */
+int fwi_copy_bandwidth(fwi_ctx* ctx, long bytes, int reps, double* gbps);
+
 /* Synchronise the context's stream. */
 int fwi_sync(fwi_ctx* ctx);
 

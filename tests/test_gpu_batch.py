@@ -27,8 +27,9 @@ def _shots(case, nshots):
     """Sources on a grid near the surface, each with a receiver patch of
     half-width 10 cells centred on it and clipped to the interior."""
     nb = case.nb
-    xs = np.linspace(nb + 4, nb + case.nx - 5, 2 if nshots > 1 else 1).astype(int)
-    ys = np.linspace(nb + 4, nb + case.ny - 5, max(1, nshots // 2)).astype(int)
+    # one column of shots near the low-x edge (clipped patches), one inside
+    xs = np.array([nb + 3, nb + case.nx // 2])[: 2 if nshots > 1 else 1]
+    ys = np.linspace(nb + 3, nb + case.ny // 2, max(1, nshots // 2)).astype(int)
     out = []
     for sy in ys:
         for sx in xs:
