@@ -32,6 +32,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of one step-kernel instance, as measured by the
+    two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, scaled by the copy
+    kernel's calibration) summarised in profiles/*_hbm_traffic.txt; None when
+    no summary covers this kernel instance."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.txt"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel_key:
+                return int(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(w, vp_pad, seconds):
     """Oracle (OpenMP, reference arithmetic) on a bounded sample of the same
     workload: a few forward steps and a few backward (RTM) steps on the full
@@ -245,8 +259,14 @@ def main():
         bwd_bytes = 2 * BYTES_PER_UPDATE * cells
         fwd_bytes = BYTES_PER_UPDATE * cells
         ach = bwd_bytes / bwd_avg_s / 1e9
+        by, rpt, _ = ctx.tuning(lib.KIND_BWD_RTM)
+        key = "step_kernel<BWD_RTM,%s>_BY=%d_RPT=%d_SB=1" % ("REF" if args.arith == "reference" else "FP32", by, rpt)
+        traffic, traffic_src = measured_traffic(key)
+        if traffic:
+            extra["traffic_source"] = traffic_src
+            extra["traffic_GBs_at_measured_bytes"] = round(traffic / bwd_avg_s / 1e9, 1)
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "frac_of_copy_kernel": round(ach / copy_gbs, 4),
                     "kernel": "step_kernel<BWD_RTM> (fused reconstruction + receiver propagation + imaging)",
                     "avg_launch_us": round(bwd_avg_s * 1e6, 2), "algorithmic_bytes_per_launch": bwd_bytes}

@@ -315,14 +315,27 @@ hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, f
     return hipGetLastError();
 }
 
-// Plain streaming copy (16-byte vectors): the calibration kernel whose
-// bandwidth and HBM counters the stencil kernels are compared with.
-__global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, long n) {
-    for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256) dst[q] = src[q];
+// Plain streaming copy: the calibration kernel whose bandwidth and HBM
+// counters the stencil kernels are compared with.  Each thread moves four
+// 16-byte vectors (all four loads issued before the stores), a block moves
+// 16 KiB, and the grid covers the buffer once.
+typedef float copy_vec __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) copy_kernel(const copy_vec* __restrict__ src, copy_vec* __restrict__ dst,
+                                                   long n) {
+    const long base = blockIdx.x * 1024L + threadIdx.x;
+    copy_vec v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (base + u * 256 < n) v[u] = __builtin_nontemporal_load(src + base + u * 256);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (base + u * 256 < n) __builtin_nontemporal_store(v[u], dst + base + u * 256);
 }
 
 hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st) {
-    copy_kernel<<<8192, 256, 0, st>>>((const float4*)src, (float4*)dst, nfloats / 4);
+    const long n = nfloats / 4;
+    copy_kernel<<<(unsigned)((n + 1023) / 1024), 256, 0, st>>>((const copy_vec*)src, (copy_vec*)dst, n);
     return hipGetLastError();
 }
 
