@@ -454,6 +454,9 @@ int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out
     if (g.nxx < 9 || g.nyy < 9 || g.nzz < 9 || g.nb < 0 || g.nt < 1 || g.tlag < 0 || !(g.dh > 0) || !(g.dt > 0))
         return fail(nullptr, FWI_ERR_ARG, "invalid grid");
     if (nshots < 1) return fail(nullptr, FWI_ERR_ARG, "nshots must be >= 1");
+    // the step kernels address a volume with 32-bit byte offsets below 2^31
+    if (fwi::make_layout(g.nxx, g.nyy, g.nzz).vol * 4 >= (1L << 31))
+        return fail(nullptr, FWI_ERR_RANGE, "padded volume exceeds 2 GiB");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, FWI_ERR_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return fail(nullptr, FWI_ERR_ARG, "device index out of range");

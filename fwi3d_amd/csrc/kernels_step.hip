@@ -101,6 +101,32 @@ __device__ __forceinline__ float damper(DampAxis i, DampAxis j, DampAxis k) {
     return ((t1 + t2) + t3) - 5.0f;
 }
 
+// Buffer loads and stores.  Every volume of a launch is addressed through a
+// wave-uniform buffer resource (base, size in bytes) held in scalar
+// registers; a lane supplies one 32-bit in-plane byte offset and the plane
+// offset is a scalar.  Lanes with nothing to load or store use the offset
+// OOB, which the hardware range check turns into a load of 0 and a dropped
+// store: the loop body has no per-lane branches, so the compiler can keep
+// several planes of loads and stores in flight with exact wait counts.
+// Volumes are below 2 GiB (checked when the context is created), so an
+// in-range offset plus the plane offset never reaches OOB.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr unsigned OOB = 0x80000000u;
+constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer resource word 3: 32-bit data, raw addressing
+constexpr int kNT = 2;                    // cache policy: non-temporal stream
+
+__device__ __forceinline__ Rsrc make_rsrc(const float* p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kBufferFlags);
+}
+template <int AUX = 0>
+__device__ __forceinline__ float bload(Rsrc r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+}
+template <int AUX = 0>
+__device__ __forceinline__ void bstore(Rsrc r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, AUX);
+}
+
 template <int MODE, int ARITH, int BY, int RPT, int SB>
 __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
@@ -131,6 +157,9 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             }
         }
     }
+
+    const long vol_b = L.vol * 4;                  // bytes of one volume (< 2 GiB)
+    const unsigned plane_b = (unsigned)(L.pp * 4);  // bytes of one z-plane
 
     // XCD-aware item order: workgroups are dispatched round-robin over the
     // 8 XCDs, so blocks b, b+8, b+16, ... share an L2.  Give each XCD a
@@ -163,43 +192,62 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
         const bool hx_lane = tx < 4 || tx >= 60;
         const bool hxok = hx_lane && hxx >= 0 && hxx < L.nx;
 
-        int yr[RPT];
-        bool act[RPT], yld[RPT];
+        // in-plane byte offsets: qo = stencil-field load (any cell of the
+        // volume), co = updated cell (loads of the streams, stores), hxo /
+        // hyo = halo cells; OOB where the lane has no such cell
+        unsigned qo[RPT], co[RPT], hxo[RPT];
+        int cell[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            yr[r] = y0 + ty + r * BY;
-            act[r] = xin && yr[r] < L.ny - 4;
-            yld[r] = xld && yr[r] < L.ny;
+            const int yr = y0 + ty + r * BY;
+            cell[r] = x + yr * L.px;
+            qo[r] = (xld && yr < L.ny) ? (unsigned)cell[r] * 4u : OOB;
+            co[r] = (xin && yr < L.ny - 4) ? (unsigned)cell[r] * 4u : OOB;
+            hxo[r] = (hxok && yr < L.ny) ? (unsigned)(hxx + yr * L.px) * 4u : OOB;
         }
-        int hy_y[HY], hy_row[HY];
-        bool hy_ok[HY];
+        int hy_row[HY];
+        unsigned hyo[HY];
 #pragma unroll
         for (int m = 0; m < HY; ++m) {
             const int h = ty + m * BY;
-            hy_y[m] = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
+            const int yy = h < 4 ? y0 - 4 + h : y0 + TY + (h - 4);
             hy_row[m] = h < 4 ? h : TY + h;
-            hy_ok[m] = h < 8 && xld && hy_y[m] < L.ny;
+            hyo[m] = (h < 8 && xld && yy < L.ny) ? (unsigned)(x + yy * L.px) * 4u : OOB;
         }
 
         DampAxis dmp_x{0.0f, false}, dmp_y[RPT];
         if constexpr (ABC) {
             dmp_x = damp_axis(a.b1d, a.nb, x, L.nx);
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) dmp_y[r] = damp_axis(a.b1d, a.nb, yr[r], L.ny);
+            for (int r = 0; r < RPT; ++r) dmp_y[r] = damp_axis(a.b1d, a.nb, y0 + ty + r * BY, L.ny);
         }
 
-        const float* fld[SB][NF];
-        long sbase[SB];
-        long src_off[SB];
+        // per-shot buffer resources (wave-uniform)
+        Rsrc fb[SB][NF], ob[SB], vb[SB], rb[SB];
+        int src_z[SB], src_c[SB];
 #pragma unroll
         for (int s = 0; s < SB; ++s) {
             const int shot = grp * SB + s;
-            sbase[s] = (long)shot * a.sstride;
-            fld[s][0] = a.cur + sbase[s];
-            if constexpr (BWD) fld[s][1] = a.rcur + sbase[s];
-            src_off[s] = BWD ? -1 : a.src_off[shot];
+            const long sb = (long)shot * a.sstride;
+            fb[s][0] = make_rsrc(a.cur + sb, vol_b);
+            ob[s] = make_rsrc(a.old + sb, vol_b);
+            vb[s] = make_rsrc(a.vp + sb, vol_b);
+            if constexpr (BWD) {
+                fb[s][1] = make_rsrc(a.rcur + sb, vol_b);
+                rb[s] = make_rsrc(a.rold + sb, vol_b);
+            }
+            src_z[s] = -1;
+            src_c[s] = -1;
+            if constexpr (!BWD) {
+                const long so = a.src_off[shot];
+                if (so >= 0) {
+                    src_z[s] = (int)(so / L.pp);
+                    src_c[s] = (int)(so - (long)src_z[s] * L.pp);
+                }
+            }
         }
-        const long gbase = (long)grp * a.istride;
+        Rsrc ib = make_rsrc(a.image + (long)grp * a.istride, vol_b);
+        Rsrc lb = make_rsrc(a.illum + (long)grp * a.istride, vol_b);
 
         // Pipeline registers: q = z-samples z-4..z+4 of the stencil fields;
         // hx/hy = halo values of the next plane; o/v/ro/im/il = per-cell
@@ -210,50 +258,72 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
         float hy[SB][NF][HY];
         float o[SB][RPT], v[SB][RPT], ro[SB][RPT], im[RPT], il[RPT];
         float o_n[SB][RPT], v_n[SB][RPT], ro_n[SB][RPT], im_n[RPT], il_n[RPT];
+        // results of the plane just computed
+        float res[SB][NF][RPT], img_r[RPT], ill_r[RPT];
 
         auto load_plane = [&](int zz, float (&oo)[SB][RPT], float (&vv)[SB][RPT], float (&rr)[SB][RPT],
                               float (&ii)[RPT], float (&ll)[RPT]) {
             // halos of plane zz and the streamed per-cell values of plane zz
+            const unsigned pz = (unsigned)zz * plane_b;
 #pragma unroll
             for (int s = 0; s < SB; ++s)
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
 #pragma unroll
-                    for (int r = 0; r < RPT; ++r)
-                        hx[s][f][r] = (hxok && yr[r] < L.ny) ? fld[s][f][L.at(hxx, yr[r], zz)] : 0.0f;
+                    for (int r = 0; r < RPT; ++r) hx[s][f][r] = bload(fb[s][f], hxo[r], pz);
 #pragma unroll
-                    for (int m = 0; m < HY; ++m) hy[s][f][m] = hy_ok[m] ? fld[s][f][L.at(x, hy_y[m], zz)] : 0.0f;
+                    for (int m = 0; m < HY; ++m)  // wave-uniform: waves holding y-halo rows
+                        if (ty + m * BY < 8) hy[s][f][m] = bload(fb[s][f], hyo[m], pz);
                 }
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                const long off = L.at(x, yr[r], zz);
 #pragma unroll
                 for (int s = 0; s < SB; ++s) {
-                    oo[s][r] = act[r] ? __builtin_nontemporal_load(a.old + sbase[s] + off) : 0.0f;
-                    vv[s][r] = act[r] ? __builtin_nontemporal_load(a.vp + sbase[s] + off) : 0.0f;
-                    if constexpr (BWD) rr[s][r] = act[r] ? __builtin_nontemporal_load(a.rold + sbase[s] + off) : 0.0f;
+                    oo[s][r] = bload<kNT>(ob[s], co[r], pz);
+                    vv[s][r] = bload<kNT>(vb[s], co[r], pz);
+                    if constexpr (BWD) rr[s][r] = bload<kNT>(rb[s], co[r], pz);
                 }
                 if constexpr (BWD) {
-                    ii[r] = act[r] ? __builtin_nontemporal_load(a.image + gbase + off) : 0.0f;
-                    ll[r] = act[r] ? __builtin_nontemporal_load(a.illum + gbase + off) : 0.0f;
+                    ii[r] = bload<kNT>(ib, co[r], pz);
+                    ll[r] = bload<kNT>(lb, co[r], pz);
                 }
             }
         };
         auto stage_plane = [&](int buf, int qi) {
-            // centre values (queue slot qi) and the halos into LDS buffer buf
+            // Centre values (queue slot qi) and the halos into LDS buffer
+            // buf.  x halos without a per-lane branch (a lane with no halo
+            // cell rewrites its own centre value); y-halo rows belong to
+            // whole waves, so that test is wave-uniform.
 #pragma unroll
             for (int s = 0; s < SB; ++s)
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
+                    float(*T)[LW] = lds[buf][s * NF + f];
 #pragma unroll
                     for (int r = 0; r < RPT; ++r) {
-                        lds[buf][s * NF + f][ty + r * BY + 4][tx + 4] = q[s][f][r][qi];
-                        if (hx_lane) lds[buf][s * NF + f][ty + r * BY + 4][hxc] = hx[s][f][r];
+                        const float cv = q[s][f][r][qi];
+                        T[ty + r * BY + 4][tx + 4] = cv;
+                        T[ty + r * BY + 4][hx_lane ? hxc : tx + 4] = hx_lane ? hx[s][f][r] : cv;
                     }
 #pragma unroll
                     for (int m = 0; m < HY; ++m)
-                        if (ty + m * BY < 8) lds[buf][s * NF + f][hy_row[m]][tx + 4] = hy[s][f][m];
+                        if (ty + m * BY < 8) T[hy_row[m]][tx + 4] = hy[s][f][m];
                 }
+        };
+        auto store_plane = [&](int zz) {
+            const unsigned pz = (unsigned)zz * plane_b;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+                for (int s = 0; s < SB; ++s) {
+                    bstore<kNT>(ob[s], co[r], pz, res[s][0][r]);
+                    if constexpr (BWD) bstore<kNT>(rb[s], co[r], pz, res[s][1][r]);
+                }
+                if constexpr (BWD) {
+                    bstore<kNT>(lb, co[r], pz, ill_r[r]);
+                    bstore<kNT>(ib, co[r], pz, img_r[r]);
+                }
+            }
         };
 
         // prologue: planes zb-4 .. zb+4 into the queue, plane zb into LDS
@@ -262,36 +332,36 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
 #pragma unroll
             for (int f = 0; f < NF; ++f)
 #pragma unroll
-                for (int r = 0; r < RPT; ++r)
+                for (int d = 0; d < 9; ++d)
 #pragma unroll
-                    for (int d = 0; d < 9; ++d)
-                        q[s][f][r][d] = yld[r] ? fld[s][f][L.at(x, yr[r], zb - 4 + d)] : 0.0f;
+                    for (int r = 0; r < RPT; ++r)
+                        q[s][f][r][d] = bload(fb[s][f], qo[r], (unsigned)(zb - 4 + d) * plane_b);
         load_plane(zb, o, v, ro, im, il);
         stage_plane(0, 4);
         __syncthreads();
 
         for (int z = zb; z < ze; ++z) {
             const int buf = (z - zb) & 1;
-            const bool more = z + 1 < ze;  // uniform per block
-            // 1. prefetch plane z+1 (queue head z+5, halos, streams)
-            if (more) {
+            // 1. prefetch plane z+1: queue head z+5, halos, streams.  The
+            // loop body is one straight block (no branch on the last plane,
+            // whose prefetched values are simply unused; the queue head is
+            // clamped to the last plane of the volume there).
+            {
+                const unsigned ph = (unsigned)min(z + 5, L.nz - 1) * plane_b;
 #pragma unroll
                 for (int s = 0; s < SB; ++s)
 #pragma unroll
                     for (int f = 0; f < NF; ++f)
 #pragma unroll
-                        for (int r = 0; r < RPT; ++r)
-                            qn[s][f][r] = yld[r] ? fld[s][f][L.at(x, yr[r], z + 5)] : 0.0f;
+                        for (int r = 0; r < RPT; ++r) qn[s][f][r] = bload(fb[s][f], qo[r], ph);
                 load_plane(z + 1, o_n, v_n, ro_n, im_n, il_n);
             }
 
-            // 2. compute plane z
+            // 2. compute plane z (results kept in registers)
             DampAxis dmp_z{0.0f, false};
             if constexpr (ABC) dmp_z = damp_axis(a.b1d, a.nb, z, L.nz);
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                if (!act[r]) continue;
-                const long off = L.at(x, yr[r], z);
                 const int lr = ty + r * BY + 4;
                 const int lc = tx + 4;
                 float ill = il[r], img = im[r];
@@ -319,8 +389,8 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                         nv = nv + 2.0f * c;
                         nv = nv - oo;
                         if constexpr (ABC) nv = nv * dmp;
-                        if (off == src_off[s]) nv = nv + a.inj;
-                        __builtin_nontemporal_store(nv, a.old + sbase[s] + off);
+                        if (z == src_z[s] && cell[r] == src_c[s]) nv = nv + a.inj;
+                        res[s][0][r] = nv;
                     } else {
                         const float(*R)[LW] = lds[buf][s * NF + 1];
                         const float cr = q[s][1][r][4];
@@ -339,8 +409,8 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                         float rn = K * ((rxx + ryy) + rzz);
                         rn = rn + 2.0f * cr;
                         rn = rn - ro[s][r];
-                        __builtin_nontemporal_store(sn, a.old + sbase[s] + off);
-                        __builtin_nontemporal_store(rn, a.rold + sbase[s] + off);
+                        res[s][0][r] = sn;
+                        res[s][1][r] = rn;
                         ill = ill + sn * sn;
                         if constexpr (MODE == BWD_RTM) {
                             img = img + sn * cr;
@@ -352,38 +422,40 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
                         }
                     }
                 }
-                if constexpr (BWD) {
-                    __builtin_nontemporal_store(ill, a.illum + gbase + off);
-                    __builtin_nontemporal_store(img, a.image + gbase + off);
-                }
+                ill_r[r] = ill;
+                img_r[r] = img;
             }
 
-            // 3. stage plane z+1 into the other LDS buffer, advance the queue
-            if (more) {
-                stage_plane(buf ^ 1, 5);
+            // Keep the compute above this point: staging waits for the
+            // prefetched loads, which must stay in flight during the compute.
+            __builtin_amdgcn_sched_barrier(0);
+
+            // 3. store plane z; stage plane z+1 into the other LDS buffer
+            // (the staging waits only for the prefetched loads), advance the queue
+            store_plane(z);
+            stage_plane(buf ^ 1, 5);
 #pragma unroll
-                for (int s = 0; s < SB; ++s) {
+            for (int s = 0; s < SB; ++s) {
 #pragma unroll
-                    for (int f = 0; f < NF; ++f)
-#pragma unroll
-                        for (int r = 0; r < RPT; ++r) {
-#pragma unroll
-                            for (int d = 0; d < 8; ++d) q[s][f][r][d] = q[s][f][r][d + 1];
-                            q[s][f][r][8] = qn[s][f][r];
-                        }
+                for (int f = 0; f < NF; ++f)
 #pragma unroll
                     for (int r = 0; r < RPT; ++r) {
-                        o[s][r] = o_n[s][r];
-                        v[s][r] = v_n[s][r];
-                        if constexpr (BWD) ro[s][r] = ro_n[s][r];
+#pragma unroll
+                        for (int d = 0; d < 8; ++d) q[s][f][r][d] = q[s][f][r][d + 1];
+                        q[s][f][r][8] = qn[s][f][r];
                     }
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    o[s][r] = o_n[s][r];
+                    v[s][r] = v_n[s][r];
+                    if constexpr (BWD) ro[s][r] = ro_n[s][r];
                 }
-                if constexpr (BWD) {
+            }
+            if constexpr (BWD) {
 #pragma unroll
-                    for (int r = 0; r < RPT; ++r) {
-                        im[r] = im_n[r];
-                        il[r] = il_n[r];
-                    }
+                for (int r = 0; r < RPT; ++r) {
+                    im[r] = im_n[r];
+                    il[r] = il_n[r];
                 }
             }
             __syncthreads();

@@ -36,7 +36,7 @@ extern "C" {
 #define FWI_ERR_HIP (-2)       /* a HIP runtime call failed */
 #define FWI_ERR_CFL (-3)       /* model violates v*dt/dh <= 0.452856 */
 #define FWI_ERR_STATE (-4)     /* call order: e.g. backward before forward */
-#define FWI_ERR_RANGE (-5)     /* source/receiver outside the updated grid */
+#define FWI_ERR_RANGE (-5)     /* source/receiver outside the updated grid; padded volume >= 2 GiB */
 #define FWI_ERR_NONFINITE (-6) /* NaN/Inf found in a model or wavefield */
 #define FWI_ERR_NODEV (-7)     /* no HIP device */
 
