@@ -96,6 +96,7 @@ def batched_rtm(w, single, nshots, lib, configs):
     c.autotune(lib.KIND_BWD_RTM, 3)
     log("batched autotune %.1f s: forward %s sb=%d, rtm %s sb=%d" % (
         time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.tuning(2), c.shots_per_block(2)))
+    c.set_graph(True)  # each pass's time loop is one captured hipGraph
     c.forward(lib.BOUNDARY_RBC)
     data = [c.seismogram(slot=s) for s in range(S)]
     for s in range(S):
@@ -161,7 +162,10 @@ def main():
                     help="profiling only: one 1 GiB copy-kernel launch (counter calibration)")
     ap.add_argument("--tuning", default="",
                     help="fixed launch shapes instead of autotuning, e.g. 'fwd=4,2,8;rtm=16,1,1'")
-    ap.add_argument("--batch-shots", type=int, default=4,
+    ap.add_argument("--fwi-scale", default="small", choices=["none", "tiny", "small", "medium"],
+                    help="also run plain FWI against learned-preconditioned FWI (time to equal misfit) at this "
+                         "grid scale (fwi3d_amd.experiment)")
+    ap.add_argument("--batch-shots", type=int, default=16,
                     help="also time this many shots of the 4x4 surface survey as one batch against the "
                          "per-shot path (0: skip; 16: the full survey)")
     args = ap.parse_args()
@@ -274,17 +278,37 @@ def main():
                                    "achieved_GBs": round(fwd_bytes / fwd_avg_s / 1e9, 1),
                                    "frac": round(fwd_bytes / fwd_avg_s / 1e9 / HBM_PEAK_GBS, 4)}
     pct_hbm = value * BYTES_PER_UPDATE / HBM_PEAK_GBS
+    # the timed shot's padded model (the per-shot leg below replaces its boundary)
+    ctx_vp = ctx.get_volume(lib.VOL_MODEL) if not args.no_cpu_baseline else None
 
     batched = None
     if args.batch_shots > 0 and not args.pass_steps:
         batched = batched_rtm(w, ctx, args.batch_shots, lib, configs)
+
+    fwi_res = None
+    if args.fwi_scale != "none" and not args.pass_steps:
+        from fwi3d_amd import experiment
+        log("FWI: plain vs learned preconditioner, scale %s ..." % args.fwi_scale)
+        r = experiment.run(args.fwi_scale, 0, log=log)
+        fwi_res = {"scale": r["scale"], "grid": r["grid"], "shots": r["shots"], "nt": r["nt"],
+                   "time_to_misfit_ratio": r["time_to_misfit_ratio"],
+                   "plain_5_iterations_s": r["plain"]["seconds"], "learned_to_same_misfit_s": r["learned"]["seconds"],
+                   "plain_misfit": [round(v, 2) for v in r["plain"]["misfit"]],
+                   "learned_misfit": [round(v, 2) for v in r["learned"]["misfit"]],
+                   "learned_updates": r["learned"]["updates"],
+                   "model_rel_l2": {"initial": round(r["plain"]["initial_model_rel_l2"], 4),
+                                    "plain": round(r["plain"]["model_rel_l2"], 4),
+                                    "learned": round(r["learned"]["model_rel_l2"], 4)},
+                   "training": {"triples": r["training_triples"], "triples_s": r["triples_seconds"],
+                                "train_s": r["train_seconds"], "loss_first_last": r["train_loss_first_last"]},
+                   "note": "wall time of each FWI run (every forward/backward pass, line search, network inference "
+                           "and fallback); training data generation and training are reported separately"}
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         import oracle as O
         vp_pad = O.random_boundary(O.expand_boundary(w.interior, w.nb), w.nb, w.rbc_seed, 0, w.rbc_amp, w.vclamp)
-        ctx_vp = ctx.get_volume(lib.VOL_MODEL)
         if not np.array_equal(ctx_vp, vp_pad):
             log("warning: device model differs from the oracle's")
         ctx.close()
@@ -316,6 +340,7 @@ def main():
         "forward_pass_ms": round(fwd_pass / args.steps, 2),
         "rtm_backward_pass_ms": round(bwd_pass / args.steps, 2),
         "batched_rtm": batched,
+        "fwi_time_to_misfit": fwi_res,
         "copy_kernel_GBs": round(copy_gbs, 1),
     }
     out.update(extra)

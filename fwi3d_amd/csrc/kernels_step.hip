@@ -26,8 +26,12 @@
 //    the current field (the reference's P *= damper, which races with the
 //    neighbours' reads) is applied instead when the cell is next read as
 //    "old", giving the same values in a single pass.
-//  * SB shots per block (batched mode) share the item's image/illumination
-//    read-modify-write.
+//  * batched launches put SB shots of the same tile in one block, one shot
+//    per group of BY waves (so each thread carries one shot's registers, as
+//    in a single-shot launch); in the backward modes only group 0 reads and
+//    writes image and illumination, after adding the other groups' terms
+//    of the plane, which they pass through LDS: that read-modify-write is
+//    shared by the SB shots.
 //
 // Arithmetic.  ARITH_REF reproduces the reference kernel literally: float
 // pair sums, double coefficients and accumulation per axis, rounding to
@@ -128,7 +132,7 @@ __device__ __forceinline__ void bstore(Rsrc r, unsigned voff, unsigned soff, flo
 }
 
 template <int MODE, int ARITH, int BY, int RPT, int SB>
-__global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
+__global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
     constexpr bool ABC = MODE == FWD_ABC;
@@ -137,9 +141,17 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
     constexpr int LH = TY + 8;
     constexpr int HY = (8 + BY - 1) / BY;  // y-halo rows loaded per wave
     __shared__ float lds[2][SB * NF][LH][LW];
+    // batched backward: the image/illumination terms of groups 1..SB-1,
+    // double-buffered by plane parity, summed by group 0
+    constexpr int NX = (BWD && SB > 1) ? 2 * (SB - 1) : 1;
+    __shared__ float xch[2][NX][TY][64];
 
     const Layout& L = a.L;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x;
+    // wave group = shot within the block (wave-uniform: kept in a scalar register)
+    const int g = SB == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.y / BY);
+    // wave row within the group's tile (wave-uniform)
+    const int ty = __builtin_amdgcn_readfirstlane(SB == 1 ? (int)threadIdx.y : (int)threadIdx.y % BY);
 
     // Forward receiver gather of the current field (compute_seismogram,
     // modeling.cu:384-393).  The current field is read-only in this launch
@@ -150,7 +162,7 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             const int per = (nrec + gridDim.x - 1) / gridDim.x;
             const int beg = blockIdx.x * per;
             const int end = min(beg + per, nrec);
-            for (int q = beg + ty * 64 + tx; q < end; q += 64 * BY) {
+            for (int q = beg + threadIdx.y * 64 + tx; q < end; q += 64 * BY * SB) {
                 float v = a.cur[a.rec_off[q]];
                 if (a.rec_damp) v = v * a.rec_damp[q];
                 a.seis_row[q] = v;
@@ -222,70 +234,69 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             for (int r = 0; r < RPT; ++r) dmp_y[r] = damp_axis(a.b1d, a.nb, y0 + ty + r * BY, L.ny);
         }
 
-        // per-shot buffer resources (wave-uniform)
-        Rsrc fb[SB][NF], ob[SB], vb[SB], rb[SB];
-        int src_z[SB], src_c[SB];
-#pragma unroll
-        for (int s = 0; s < SB; ++s) {
-            const int shot = grp * SB + s;
-            const long sb = (long)shot * a.sstride;
-            fb[s][0] = make_rsrc(a.cur + sb, vol_b);
-            ob[s] = make_rsrc(a.old + sb, vol_b);
-            vb[s] = make_rsrc(a.vp + sb, vol_b);
-            if constexpr (BWD) {
-                fb[s][1] = make_rsrc(a.rcur + sb, vol_b);
-                rb[s] = make_rsrc(a.rold + sb, vol_b);
-            }
-            src_z[s] = -1;
-            src_c[s] = -1;
-            if constexpr (!BWD) {
-                const long so = a.src_off[shot];
-                if (so >= 0) {
-                    src_z[s] = (int)(so / L.pp);
-                    src_c[s] = (int)(so - (long)src_z[s] * L.pp);
-                }
+        // this group's shot: buffer resources (wave-uniform)
+        const int shot = grp * SB + g;
+        const long sbase = (long)shot * a.sstride;
+        Rsrc fb[NF], ob, vb, rb;
+        fb[0] = make_rsrc(a.cur + sbase, vol_b);
+        ob = make_rsrc(a.old + sbase, vol_b);
+        vb = make_rsrc(a.vp + sbase, vol_b);
+        if constexpr (BWD) {
+            fb[1] = make_rsrc(a.rcur + sbase, vol_b);
+            rb = make_rsrc(a.rold + sbase, vol_b);
+        }
+        int src_z = -1, src_c = -1;
+        if constexpr (!BWD) {
+            const long so = a.src_off[shot];
+            if (so >= 0) {
+                src_z = (int)(so / L.pp);
+                src_c = (int)(so - (long)src_z * L.pp);
             }
         }
+        // image/illumination of the shot group: read and written by group 0
         Rsrc ib = make_rsrc(a.image + (long)grp * a.istride, vol_b);
         Rsrc lb = make_rsrc(a.illum + (long)grp * a.istride, vol_b);
+        const bool owner = g == 0;
 
         // Pipeline registers: q = z-samples z-4..z+4 of the stencil fields;
         // hx/hy = halo values of the next plane; o/v/ro/im/il = per-cell
         // streams of the plane being computed, *_n = of the next plane.
-        float q[SB][NF][RPT][9];
-        float qn[SB][NF][RPT];
-        float hx[SB][NF][RPT];
-        float hy[SB][NF][HY];
-        float o[SB][RPT], v[SB][RPT], ro[SB][RPT], im[RPT], il[RPT];
-        float o_n[SB][RPT], v_n[SB][RPT], ro_n[SB][RPT], im_n[RPT], il_n[RPT];
-        // results of the plane just computed
-        float res[SB][NF][RPT], img_r[RPT], ill_r[RPT];
+        float q[NF][RPT][9];
+        float qn[NF][RPT];
+        float hx[NF][RPT];
+        float hy[NF][HY];
+        float o[RPT], v[RPT], ro[RPT], im[RPT], il[RPT];
+        float o_n[RPT], v_n[RPT], ro_n[RPT], im_n[RPT], il_n[RPT];
+        // results of the plane just computed; img_r/ill_r: image and
+        // illumination with this group's term added (group 0: the running
+        // sums, completed with the other groups' terms one plane later)
+        float res[NF][RPT], img_r[RPT], ill_r[RPT];
 
-        auto load_plane = [&](int zz, float (&oo)[SB][RPT], float (&vv)[SB][RPT], float (&rr)[SB][RPT],
-                              float (&ii)[RPT], float (&ll)[RPT]) {
+        auto load_plane = [&](int zz, float (&oo)[RPT], float (&vv)[RPT], float (&rr)[RPT], float (&ii)[RPT],
+                              float (&ll)[RPT]) {
             // halos of plane zz and the streamed per-cell values of plane zz
             const unsigned pz = (unsigned)zz * plane_b;
 #pragma unroll
-            for (int s = 0; s < SB; ++s)
+            for (int f = 0; f < NF; ++f) {
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
+                for (int r = 0; r < RPT; ++r) hx[f][r] = bload(fb[f], hxo[r], pz);
 #pragma unroll
-                    for (int r = 0; r < RPT; ++r) hx[s][f][r] = bload(fb[s][f], hxo[r], pz);
-#pragma unroll
-                    for (int m = 0; m < HY; ++m)  // wave-uniform: waves holding y-halo rows
-                        if (ty + m * BY < 8) hy[s][f][m] = bload(fb[s][f], hyo[m], pz);
-                }
+                for (int m = 0; m < HY; ++m)  // wave-uniform: waves holding y-halo rows
+                    if (ty + m * BY < 8) hy[f][m] = bload(fb[f], hyo[m], pz);
+            }
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
+                oo[r] = bload<kNT>(ob, co[r], pz);
+                vv[r] = bload<kNT>(vb, co[r], pz);
+                if constexpr (BWD) rr[r] = bload<kNT>(rb, co[r], pz);
+            }
+            if constexpr (BWD) {
+                if (owner) {
 #pragma unroll
-                for (int s = 0; s < SB; ++s) {
-                    oo[s][r] = bload<kNT>(ob[s], co[r], pz);
-                    vv[s][r] = bload<kNT>(vb[s], co[r], pz);
-                    if constexpr (BWD) rr[s][r] = bload<kNT>(rb[s], co[r], pz);
-                }
-                if constexpr (BWD) {
-                    ii[r] = bload<kNT>(ib, co[r], pz);
-                    ll[r] = bload<kNT>(lb, co[r], pz);
+                    for (int r = 0; r < RPT; ++r) {
+                        ii[r] = bload<kNT>(ib, co[r], pz);
+                        ll[r] = bload<kNT>(lb, co[r], pz);
+                    }
                 }
             }
         };
@@ -295,47 +306,45 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             // cell rewrites its own centre value); y-halo rows belong to
             // whole waves, so that test is wave-uniform.
 #pragma unroll
-            for (int s = 0; s < SB; ++s)
+            for (int f = 0; f < NF; ++f) {
+                float(*T)[LW] = lds[buf][g * NF + f];
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    float(*T)[LW] = lds[buf][s * NF + f];
-#pragma unroll
-                    for (int r = 0; r < RPT; ++r) {
-                        const float cv = q[s][f][r][qi];
-                        T[ty + r * BY + 4][tx + 4] = cv;
-                        T[ty + r * BY + 4][hx_lane ? hxc : tx + 4] = hx_lane ? hx[s][f][r] : cv;
-                    }
-#pragma unroll
-                    for (int m = 0; m < HY; ++m)
-                        if (ty + m * BY < 8) T[hy_row[m]][tx + 4] = hy[s][f][m];
+                for (int r = 0; r < RPT; ++r) {
+                    const float cv = q[f][r][qi];
+                    T[ty + r * BY + 4][tx + 4] = cv;
+                    T[ty + r * BY + 4][hx_lane ? hxc : tx + 4] = hx_lane ? hx[f][r] : cv;
                 }
+#pragma unroll
+                for (int m = 0; m < HY; ++m)
+                    if (ty + m * BY < 8) T[hy_row[m]][tx + 4] = hy[f][m];
+            }
         };
-        auto store_plane = [&](int zz) {
+        // group 0: add the other groups' terms of plane zz (parity buffer
+        // xb), in shot order, and store image and illumination
+        auto finish_plane = [&](int zz, int xb) {
             const unsigned pz = (unsigned)zz * plane_b;
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
+                float ill = ill_r[r], img = img_r[r];
+                if constexpr (SB > 1) {
 #pragma unroll
-                for (int s = 0; s < SB; ++s) {
-                    bstore<kNT>(ob[s], co[r], pz, res[s][0][r]);
-                    if constexpr (BWD) bstore<kNT>(rb[s], co[r], pz, res[s][1][r]);
+                    for (int k = 0; k < SB - 1; ++k) {
+                        ill = ill + xch[xb][2 * k][ty + r * BY][tx];
+                        img = img + xch[xb][2 * k + 1][ty + r * BY][tx];
+                    }
                 }
-                if constexpr (BWD) {
-                    bstore<kNT>(lb, co[r], pz, ill_r[r]);
-                    bstore<kNT>(ib, co[r], pz, img_r[r]);
-                }
+                bstore<kNT>(lb, co[r], pz, ill);
+                bstore<kNT>(ib, co[r], pz, img);
             }
         };
 
         // prologue: planes zb-4 .. zb+4 into the queue, plane zb into LDS
 #pragma unroll
-        for (int s = 0; s < SB; ++s)
+        for (int f = 0; f < NF; ++f)
 #pragma unroll
-            for (int f = 0; f < NF; ++f)
+            for (int d = 0; d < 9; ++d)
 #pragma unroll
-                for (int d = 0; d < 9; ++d)
-#pragma unroll
-                    for (int r = 0; r < RPT; ++r)
-                        q[s][f][r][d] = bload(fb[s][f], qo[r], (unsigned)(zb - 4 + d) * plane_b);
+                for (int r = 0; r < RPT; ++r) q[f][r][d] = bload(fb[f], qo[r], (unsigned)(zb - 4 + d) * plane_b);
         load_plane(zb, o, v, ro, im, il);
         stage_plane(0, 4);
         __syncthreads();
@@ -349,12 +358,16 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             {
                 const unsigned ph = (unsigned)min(z + 5, L.nz - 1) * plane_b;
 #pragma unroll
-                for (int s = 0; s < SB; ++s)
+                for (int f = 0; f < NF; ++f)
 #pragma unroll
-                    for (int f = 0; f < NF; ++f)
-#pragma unroll
-                        for (int r = 0; r < RPT; ++r) qn[s][f][r] = bload(fb[s][f], qo[r], ph);
+                    for (int r = 0; r < RPT; ++r) qn[f][r] = bload(fb[f], qo[r], ph);
                 load_plane(z + 1, o_n, v_n, ro_n, im_n, il_n);
+            }
+
+            // batched backward: complete plane z-1 (its terms were
+            // exchanged before the barrier that ended the previous plane)
+            if constexpr (BWD && SB > 1) {
+                if (owner && z > zb) finish_plane(z - 1, buf ^ 1);
             }
 
             // 2. compute plane z (results kept in registers)
@@ -364,66 +377,64 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
             for (int r = 0; r < RPT; ++r) {
                 const int lr = ty + r * BY + 4;
                 const int lc = tx + 4;
-                float ill = il[r], img = im[r];
-#pragma unroll
-                for (int s = 0; s < SB; ++s) {
-                    const float(*P)[LW] = lds[buf][s * NF];
-                    const float c = q[s][0][r][4];
-                    const float dxx = d2<ARITH>(c, P[lr][lc - 1], P[lr][lc + 1], P[lr][lc - 2], P[lr][lc + 2],
-                                                P[lr][lc - 3], P[lr][lc + 3], P[lr][lc - 4], P[lr][lc + 4], a.idh2);
-                    const float dyy = d2<ARITH>(c, P[lr - 1][lc], P[lr + 1][lc], P[lr - 2][lc], P[lr + 2][lc],
-                                                P[lr - 3][lc], P[lr + 3][lc], P[lr - 4][lc], P[lr + 4][lc], a.idh2);
-                    const float* qz = q[s][0][r];
-                    const float dzz = d2<ARITH>(c, qz[3], qz[5], qz[2], qz[6], qz[1], qz[7], qz[0], qz[8], a.idh2);
-                    const float vel = v[s][r];
-                    const float vp2 = vel * vel;
-                    const float K = a.dt2 * vp2;
-                    if constexpr (!BWD) {
-                        float oo = o[s][r];
-                        float dmp = 1.0f;
-                        if constexpr (ABC) {
-                            dmp = damper(dmp_z, dmp_x, dmp_y[r]);
-                            oo = oo * dmp;
-                        }
-                        float nv = K * ((dxx + dyy) + dzz);
-                        nv = nv + 2.0f * c;
-                        nv = nv - oo;
-                        if constexpr (ABC) nv = nv * dmp;
-                        if (z == src_z[s] && cell[r] == src_c[s]) nv = nv + a.inj;
-                        res[s][0][r] = nv;
+                const float(*P)[LW] = lds[buf][g * NF];
+                const float c = q[0][r][4];
+                const float dxx = d2<ARITH>(c, P[lr][lc - 1], P[lr][lc + 1], P[lr][lc - 2], P[lr][lc + 2],
+                                            P[lr][lc - 3], P[lr][lc + 3], P[lr][lc - 4], P[lr][lc + 4], a.idh2);
+                const float dyy = d2<ARITH>(c, P[lr - 1][lc], P[lr + 1][lc], P[lr - 2][lc], P[lr + 2][lc],
+                                            P[lr - 3][lc], P[lr + 3][lc], P[lr - 4][lc], P[lr + 4][lc], a.idh2);
+                const float* qz = q[0][r];
+                const float dzz = d2<ARITH>(c, qz[3], qz[5], qz[2], qz[6], qz[1], qz[7], qz[0], qz[8], a.idh2);
+                const float vel = v[r];
+                const float vp2 = vel * vel;
+                const float K = a.dt2 * vp2;
+                if constexpr (!BWD) {
+                    float oo = o[r];
+                    float dmp = 1.0f;
+                    if constexpr (ABC) {
+                        dmp = damper(dmp_z, dmp_x, dmp_y[r]);
+                        oo = oo * dmp;
+                    }
+                    float nv = K * ((dxx + dyy) + dzz);
+                    nv = nv + 2.0f * c;
+                    nv = nv - oo;
+                    if constexpr (ABC) nv = nv * dmp;
+                    if (z == src_z && cell[r] == src_c) nv = nv + a.inj;
+                    res[0][r] = nv;
+                } else {
+                    const float(*R)[LW] = lds[buf][g * NF + 1];
+                    const float cr = q[1][r][4];
+                    const float rxx = d2<ARITH>(cr, R[lr][lc - 1], R[lr][lc + 1], R[lr][lc - 2], R[lr][lc + 2],
+                                                R[lr][lc - 3], R[lr][lc + 3], R[lr][lc - 4], R[lr][lc + 4], a.idh2);
+                    const float ryy = d2<ARITH>(cr, R[lr - 1][lc], R[lr + 1][lc], R[lr - 2][lc], R[lr + 2][lc],
+                                                R[lr - 3][lc], R[lr + 3][lc], R[lr - 4][lc], R[lr + 4][lc], a.idh2);
+                    const float* rz = q[1][r];
+                    const float rzz = d2<ARITH>(cr, rz[3], rz[5], rz[2], rz[6], rz[1], rz[7], rz[0], rz[8], a.idh2);
+                    float sn = K * ((dxx + dyy) + dzz);
+                    sn = sn + 2.0f * c;
+                    sn = sn - o[r];
+                    float rn = K * ((rxx + ryy) + rzz);
+                    rn = rn + 2.0f * cr;
+                    rn = rn - ro[r];
+                    res[0][r] = sn;
+                    res[1][r] = rn;
+                    const float t_ill = sn * sn;
+                    float t_img;
+                    if constexpr (MODE == BWD_RTM) {
+                        t_img = sn * cr;
                     } else {
-                        const float(*R)[LW] = lds[buf][s * NF + 1];
-                        const float cr = q[s][1][r][4];
-                        const float rxx = d2<ARITH>(cr, R[lr][lc - 1], R[lr][lc + 1], R[lr][lc - 2], R[lr][lc + 2],
-                                                    R[lr][lc - 3], R[lr][lc + 3], R[lr][lc - 4], R[lr][lc + 4],
-                                                    a.idh2);
-                        const float ryy = d2<ARITH>(cr, R[lr - 1][lc], R[lr + 1][lc], R[lr - 2][lc], R[lr + 2][lc],
-                                                    R[lr - 3][lc], R[lr + 3][lc], R[lr - 4][lc], R[lr + 4][lc],
-                                                    a.idh2);
-                        const float* rz = q[s][1][r];
-                        const float rzz =
-                            d2<ARITH>(cr, rz[3], rz[5], rz[2], rz[6], rz[1], rz[7], rz[0], rz[8], a.idh2);
-                        float sn = K * ((dxx + dyy) + dzz);
-                        sn = sn + 2.0f * c;
-                        sn = sn - o[s][r];
-                        float rn = K * ((rxx + ryy) + rzz);
-                        rn = rn + 2.0f * cr;
-                        rn = rn - ro[s][r];
-                        res[s][0][r] = sn;
-                        res[s][1][r] = rn;
-                        ill = ill + sn * sn;
-                        if constexpr (MODE == BWD_RTM) {
-                            img = img + sn * cr;
-                        } else {
-                            float gterm = a.dt * cr;
-                            gterm = gterm * (dxx + dzz);
-                            gterm = gterm * vp2;
-                            img = img + gterm;
-                        }
+                        float gterm = a.dt * cr;
+                        gterm = gterm * (dxx + dzz);
+                        t_img = gterm * vp2;
+                    }
+                    if (owner) {
+                        ill_r[r] = il[r] + t_ill;
+                        img_r[r] = im[r] + t_img;
+                    } else if constexpr (SB > 1) {
+                        xch[buf][2 * (g - 1)][ty + r * BY][tx] = t_ill;
+                        xch[buf][2 * (g - 1) + 1][ty + r * BY][tx] = t_img;
                     }
                 }
-                ill_r[r] = ill;
-                img_r[r] = img;
             }
 
             // Keep the compute above this point: staging waits for the
@@ -432,40 +443,47 @@ __global__ void __launch_bounds__(64 * BY) step_kernel(const StepArgs a) {
 
             // 3. store plane z; stage plane z+1 into the other LDS buffer
             // (the staging waits only for the prefetched loads), advance the queue
-            store_plane(z);
+            {
+                const unsigned pz = (unsigned)z * plane_b;
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    bstore<kNT>(ob, co[r], pz, res[0][r]);
+                    if constexpr (BWD) bstore<kNT>(rb, co[r], pz, res[1][r]);
+                }
+                if constexpr (BWD && SB == 1) finish_plane(z, 0);
+            }
             stage_plane(buf ^ 1, 5);
 #pragma unroll
-            for (int s = 0; s < SB; ++s) {
-#pragma unroll
-                for (int f = 0; f < NF; ++f)
-#pragma unroll
-                    for (int r = 0; r < RPT; ++r) {
-#pragma unroll
-                        for (int d = 0; d < 8; ++d) q[s][f][r][d] = q[s][f][r][d + 1];
-                        q[s][f][r][8] = qn[s][f][r];
-                    }
+            for (int f = 0; f < NF; ++f)
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
-                    o[s][r] = o_n[s][r];
-                    v[s][r] = v_n[s][r];
-                    if constexpr (BWD) ro[s][r] = ro_n[s][r];
+#pragma unroll
+                    for (int d = 0; d < 8; ++d) q[f][r][d] = q[f][r][d + 1];
+                    q[f][r][8] = qn[f][r];
                 }
-            }
-            if constexpr (BWD) {
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) {
+            for (int r = 0; r < RPT; ++r) {
+                o[r] = o_n[r];
+                v[r] = v_n[r];
+                if constexpr (BWD) {
+                    ro[r] = ro_n[r];
                     im[r] = im_n[r];
                     il[r] = il_n[r];
                 }
             }
             __syncthreads();
         }
+        // batched backward: complete the last plane (its terms were
+        // exchanged before the loop's last barrier)
+        if constexpr (BWD && SB > 1) {
+            if (owner) finish_plane(ze - 1, (ze - 1 - zb) & 1);
+        }
     }
 }
 
 template <int MODE, int ARITH, int BY, int RPT, int SB>
 hipError_t launch_one(const StepArgs& a, dim3 grid, hipStream_t st) {
-    step_kernel<MODE, ARITH, BY, RPT, SB><<<grid, dim3(64, BY), 0, st>>>(a);
+    step_kernel<MODE, ARITH, BY, RPT, SB><<<grid, dim3(64, BY * SB), 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -481,17 +499,17 @@ struct Entry {
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB)
 
+// batched: SB wave groups per block, one shot each (BY * SB <= 16 waves)
 #define FWI_BATCH(M)                                                                                     \
     FWI_E(M, ARITH_REF, 4, 1, 2), FWI_E(M, ARITH_REF, 4, 2, 2), FWI_E(M, ARITH_REF, 8, 1, 2),            \
-        FWI_E(M, ARITH_REF, 16, 1, 2), FWI_E(M, ARITH_REF, 4, 1, 4), FWI_E(M, ARITH_REF, 8, 1, 4),        \
-        FWI_E(M, ARITH_REF, 16, 1, 4)
+        FWI_E(M, ARITH_REF, 8, 2, 2), FWI_E(M, ARITH_REF, 2, 2, 4), FWI_E(M, ARITH_REF, 4, 1, 4),         \
+        FWI_E(M, ARITH_REF, 4, 2, 4), FWI_E(M, ARITH_REF, 2, 4, 4)
 
 const Entry kTable[] = {
     FWI_TILES(FWD_RBC, ARITH_REF, 1), FWI_TILES(FWD_ABC, ARITH_REF, 1),
     FWI_TILES(BWD_RTM, ARITH_REF, 1), FWI_TILES(BWD_FWI, ARITH_REF, 1),
     FWI_TILES(FWD_RBC, ARITH_FP32, 1), FWI_TILES(FWD_ABC, ARITH_FP32, 1),
     FWI_TILES(BWD_RTM, ARITH_FP32, 1), FWI_TILES(BWD_FWI, ARITH_FP32, 1),
-    // batched: shots per block 2 and 4
     FWI_BATCH(FWD_RBC), FWI_BATCH(FWD_ABC), FWI_BATCH(BWD_RTM), FWI_BATCH(BWD_FWI),
 };
 
@@ -520,7 +538,7 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     a.nitems = a.ntx * a.nty * a.nzc * (nshots / sb);
     const int blocks = (a.nitems + 7) / 8 * 8;
     grid = dim3(blocks);
-    block = dim3(64, t.by);
+    block = dim3(64, t.by * sb);
 }
 
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {

@@ -1,0 +1,132 @@
+"""Plain FWI against FWI with the learned preconditioner: time to equal misfit.
+
+The two FWI workloads of the path, at a size one GPU runs in minutes:
+
+* plain FWI: 16 shots on a 4 x 4 surface grid over the dipping-layer /
+  Gaussian-anomaly velocity model (the 401 x 401 x 201 configuration's
+  model family, here on a smaller grid), initial model = 3-D Gaussian
+  smoothing of the true model, `iterations` gradient steps with the
+  parabolic line search;
+* learned FWI: the U-Net preconditioner, trained on (gradient,
+  illumination, model) -> (true - current model) triples that plain FWI
+  generates on seeded random layered models, proposes each update; the
+  update is accepted only when the plain forward model's misfit drops, and
+  the iteration falls back to the plain gradient step otherwise.  It runs
+  until its misfit reaches plain FWI's final misfit.
+
+Both use the same kernels and the same batched contexts; wall times count
+every forward/backward pass, line search, network inference and fallback.
+
+    python -m fwi3d_amd.experiment --scale small --out results.json
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import fwi, learned
+
+SCALES = {
+    # grid (nx, ny, nz), boundary, dt, nt, fmax, training models, FWI
+    # iterations per training model, training steps, patch (y, x, z)
+    "tiny": dict(grid=(48, 48, 32), nb=20, dt=1e-3, nt=500, fmax=30.0, n_train=6, train_iters=2, steps=150,
+                 patch=(16, 16, 16), sigma=4.0),
+    "small": dict(grid=(96, 96, 48), nb=20, dt=1e-3, nt=800, fmax=30.0, n_train=32, train_iters=3, steps=600,
+                  patch=(32, 32, 32), sigma=8.0),
+    "medium": dict(grid=(160, 160, 80), nb=30, dt=1e-3, nt=1100, fmax=30.0, n_train=48, train_iters=3, steps=900,
+                   patch=(32, 32, 32), sigma=12.0),
+}
+
+
+def true_model(nx, ny, nz, dh=10.0, seed=1):
+    """The dipping-layer / Gaussian-anomaly model family of the 401 x 401 x
+    201 workload, scaled to this grid: linear trend 1500 -> 3500 m/s over the
+    depth, 5 seeded dipping layers of +-300 m/s, a +400 m/s Gaussian anomaly
+    (sigma = 7.5 % of the width) below the centre.  (ny, nx, nz) float32."""
+    rng = np.random.default_rng(seed)
+    x, y, z = np.arange(nx) * dh, np.arange(ny) * dh, np.arange(nz) * dh
+    Y, X, Z = np.meshgrid(y, x, z, indexing="ij")
+    lz = z[-1]
+    v = 1500.0 + 2000.0 * Z / lz
+    for l, z0 in enumerate(np.linspace(0.15, 0.85, 5) * lz):
+        ax, ay = rng.uniform(-0.1, 0.1, 2)
+        surf = z0 + ax * (X - x.mean()) + ay * (Y - y.mean())
+        v += np.where(Z >= surf, 300.0 if l % 2 == 0 else -300.0, 0.0)
+    sig = 0.075 * x[-1]
+    r2 = (X - x.mean()) ** 2 + (Y - y.mean()) ** 2 + (Z - 0.6 * lz) ** 2
+    v += 400.0 * np.exp(-0.5 * r2 / sig ** 2)
+    return np.ascontiguousarray(v.astype(np.float32))
+
+
+def problem(v_true, nb, dt, nt, fmax, nshots_side=4, seed=0):
+    """Workload + observed data (ABC modeling on the true model)."""
+    ny, nx, nz = v_true.shape
+    w = learned.survey(nx, ny, nz, 10.0, nt, dt, fmax, nshots_side=nshots_side, nb=nb, seed=seed)
+    w.interior = v_true
+    return w, learned.observed(w, v_true)
+
+
+def run(scale="small", seed=0, log=None):
+    p = SCALES[scale]
+    nx, ny, nz = p["grid"]
+    log = log or (lambda m: None)
+    out = {"scale": scale, "grid": [nx, ny, nz], "nb": p["nb"], "dt": p["dt"], "nt": p["nt"], "fmax": p["fmax"],
+           "shots": 16}
+
+    # 1. training triples from plain FWI on seeded random models
+    t0 = time.perf_counter()
+    triples = learned.make_triples(p["n_train"], nx, ny, nz, p["train_iters"], p["nt"], dt=p["dt"], fmax=p["fmax"],
+                                   seed0=1000 + 1000 * seed, sigma=p["sigma"], nb=p["nb"], log=log)
+    out["training_triples"] = len(triples)
+    out["triples_seconds"] = round(time.perf_counter() - t0, 2)
+
+    # 2. train the network on the GPU
+    from .unet import UNet
+    net = UNet(seed=seed + 7)
+    t0 = time.perf_counter()
+    losses = learned.train(net, triples, p["steps"], patch=p["patch"], batch=4, lr=1e-3, seed=seed, log=log)
+    out["train_seconds"] = round(time.perf_counter() - t0, 2)
+    out["train_loss_first_last"] = [float(np.mean(losses[:20])), float(np.mean(losses[-20:]))]
+
+    # 3. the test problem
+    v_true = true_model(nx, ny, nz, seed=seed + 1)
+    v0 = learned.smooth(v_true, p["sigma"])
+    w, obs = problem(v_true, p["nb"], p["dt"], p["nt"], p["fmax"], seed=seed)
+    prob = fwi.FwiProblem(w, obs, zmask=20.0)
+    prob.misfit(v0)  # warm-up: first launches, autotune caches
+
+    def model_err(v):
+        return float(np.linalg.norm(v - v_true) / np.linalg.norm(v_true))
+
+    v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
+    out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
+                    "initial_model_rel_l2": model_err(v0)}
+    target = hist_plain[-1]
+    v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=target, max_iterations=10, log=log)
+    out["learned"] = {"misfit": hist_l, "seconds": round(sec_l, 3), "model_rel_l2": model_err(v_l),
+                      "updates": used, "reached_target": bool(hist_l[-1] <= target)}
+    out["time_to_misfit_ratio"] = round(sec_l / sec_plain, 4) if hist_l[-1] <= target else None
+    prob.close()
+    net.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--scale", choices=sorted(SCALES), default="small")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
+    s = json.dumps(res)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(s + "\n")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -99,19 +99,32 @@ def parabola_step(a0, a1, a2, f0, f1, f2):
     return float(np.float32(0.5) * (num / den))
 
 
+def velocity_bounds(w, vlo=500.0):
+    """Velocities an update may produce: from vlo up to the 3-D scheme's
+    stability limit 0.45 dh/dt (the contexts reject models beyond it)."""
+    return vlo, 0.45 * w.dh / w.dt
+
+
 def gradient_step(prob, v, g, f0, vmin, vmax):
     """Parabolic line search along the normalised gradient in m = 1/v^2
-    (inversion.cu:291-362); returns the new velocity."""
+    (inversion.cu:291-362); returns (new velocity, step).  Two guards the
+    reference lacks keep every trial model valid: slowness^2 is bounded to
+    the velocity range of velocity_bounds, and a parabola without a minimum
+    (non-finite or non-positive vertex) takes the better trial step."""
     m = (1.0 / (v.astype(np.float32) * v.astype(np.float32))).astype(np.float32)
     dm = np.float32(1.0) / (np.float32(vmin) ** 2) - np.float32(1.0) / (np.float32(vmax) ** 2)
     a1, a2 = np.float32(0.02) * dm, np.float32(0.05) * dm
+    vlo, vhi = velocity_bounds(prob.w)
+    mlo, mhi = np.float32(1.0 / vhi ** 2), np.float32(1.0 / vlo ** 2)
 
     def trial(a):
-        return (1.0 / np.sqrt((m - np.float32(a) * g).astype(np.float32))).astype(np.float32)
+        return (1.0 / np.sqrt(np.clip(m - np.float32(a) * g, mlo, mhi))).astype(np.float32)
 
     f1 = np.sqrt(prob.misfit(trial(a1)))
     f2 = np.sqrt(prob.misfit(trial(a2)))
     step = parabola_step(0.0, a1, a2, f0, f1, f2)
+    if not np.isfinite(step) or step <= 0.0:
+        step = float(a1 if f1 < f2 else a2)
     return trial(step), step
 
 

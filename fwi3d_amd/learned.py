@@ -90,12 +90,12 @@ def observed(w, v_true):
     return out
 
 
-def make_triples(n_models, nx, ny, nz, iterations, nt, dt=1e-3, fmax=30.0, seed0=1000, sigma=6.0, log=None):
+def make_triples(n_models, nx, ny, nz, iterations, nt, dt=1e-3, fmax=30.0, seed0=1000, sigma=6.0, nb=20, log=None):
     """Plain FWI on seeded random models; returns a list of (x3, target)."""
     triples = []
     for i in range(n_models):
         v_true = random_model(seed0 + i, nx, ny, nz)
-        w = survey(nx, ny, nz, 10.0, nt, dt, fmax, seed=seed0 + i)
+        w = survey(nx, ny, nz, 10.0, nt, dt, fmax, nb=nb, seed=seed0 + i)
         w.interior = v_true
         obs = observed(w, v_true)
         v0 = smooth(v_true, sigma)
@@ -139,7 +139,7 @@ def predict(net, g, illum, v):
     """Proposed velocity update (m/s) for one model."""
     x3, sl = pad4(features(g, illum, v))
     y = net.forward(x3[:, None])[0]
-    return (y[sl] * V_SCALE).astype(np.float32)
+    return (y[sl[1:]] * V_SCALE).astype(np.float32)
 
 
 def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
@@ -148,6 +148,7 @@ def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
     Returns (v, misfits, seconds, used) where used[k] says which update was
     taken ('learned' or 'gradient')."""
     vmin, vmax = float(v0.min()), float(v0.max())
+    vlo, vhi = fwi.velocity_bounds(prob.w)
     v = v0.astype(np.float32)
     hist, used = [], []
     t0 = time.perf_counter()
@@ -158,13 +159,13 @@ def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
             break
         dv = predict(net, g, illum, v)
         # parabolic line search on the learned direction (s in {0, 0.5, 1})
-        f1 = np.sqrt(prob.misfit(np.clip(v + 0.5 * dv, 300.0, None)))
-        f2 = np.sqrt(prob.misfit(np.clip(v + 1.0 * dv, 300.0, None)))
+        f1 = np.sqrt(prob.misfit(np.clip(v + 0.5 * dv, vlo, vhi)))
+        f2 = np.sqrt(prob.misfit(np.clip(v + 1.0 * dv, vlo, vhi)))
         s = fwi.parabola_step(0.0, 0.5, 1.0, hist[-1], f1, f2)
         if not np.isfinite(s) or s <= 0:
             s = 0.5 if f1 < f2 else 1.0
         s = float(min(s, 2.0))
-        v_try = np.clip(v + s * dv, 300.0, None).astype(np.float32)
+        v_try = np.clip(v + s * dv, vlo, vhi).astype(np.float32)
         f_try, g_try, _, il_try = prob.gradient(v_try)
         if np.sqrt(f_try) < hist[-1]:
             v, g, illum = v_try, g_try, il_try
