@@ -76,11 +76,13 @@ def cpu_baseline(w, vp_pad, seconds):
                       % (nf, nbk, t_fwd + t_bwd)}
 
 
-def batched_rtm(w, single, nshots, lib, configs):
+def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=True):
     """RTM of the first `nshots` shots of the 4x4 surface survey (each shot its
     own random boundary and its own 201x201 receiver patch clipped at the
     model edges), once as one batch and once shot after shot on the
-    single-shot context; returns shots/h of both and the image difference."""
+    single-shot context; returns shots/h of both and the image difference.
+    With pass_steps (profiling only) the batch runs that many steps of each
+    pass on zero data and nothing else."""
     w16 = configs.gradient_401_16(interior=w.interior)
     S = nshots
     log("batched RTM: %d shots, receivers per shot %s" % (S, [len(w16.receivers[s]) for s in range(S)]))
@@ -92,11 +94,28 @@ def batched_rtm(w, single, nshots, lib, configs):
         c.set_source(*w16.src_index(s), slot=s)
         c.set_receivers(w16.rec_index(s), slot=s)
     t0 = time.perf_counter()
-    c.autotune(lib.KIND_FWD_RBC, 3)
-    c.autotune(lib.KIND_BWD_RTM, 3)
-    log("batched autotune %.1f s: forward %s sb=%d, rtm %s sb=%d" % (
+    if tuning:
+        kinds = {"fwd": lib.KIND_FWD_RBC, "rtm": lib.KIND_BWD_RTM}
+        for item in tuning.split(";"):
+            name, vals = item.split("=")
+            by, rpt, zc, sb = [int(v) for v in vals.split(",")]
+            c.set_shots_per_block(kinds[name.strip()], sb)
+            c.set_tuning(kinds[name.strip()], by, rpt, zc)
+    else:
+        c.autotune(lib.KIND_FWD_RBC, 3)
+        c.autotune(lib.KIND_BWD_RTM, 3)
+    log("batched tuning %.1f s: forward %s sb=%d, rtm %s sb=%d" % (
         time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.tuning(2), c.shots_per_block(2)))
-    c.set_graph(True)  # each pass's time loop is one captured hipGraph
+    if pass_steps:
+        for s in range(S):
+            c.set_backward_data(np.zeros((len(w16.receivers[s]), w.nt), np.float32), slot=s)
+        c.forward_steps(lib.BOUNDARY_RBC, 0, pass_steps)
+        c.backward_steps(0, 0, pass_steps)
+        c.sync()
+        c.close()
+        return {"shots": S, "profiling_pass_steps": pass_steps}
+    sb_rtm = c.shots_per_block(lib.KIND_BWD_RTM)
+    c.set_graph(graph)  # each pass's time loop as one captured hipGraph
     c.forward(lib.BOUNDARY_RBC)
     data = [c.seismogram(slot=s) for s in range(S)]
     for s in range(S):
@@ -137,7 +156,7 @@ def batched_rtm(w, single, nshots, lib, configs):
     rel = float(np.linalg.norm((img_b - img_s).astype(np.float64)) / np.linalg.norm(img_s.astype(np.float64)))
     log("per-shot: %.2f s; image rel-L2 batched vs per-shot %.3g" % (t_seq, rel))
     cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
-    return {"shots": S, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
+    return {"shots": S, "graph": graph, "sb_rtm": sb_rtm, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
             "shots_per_hour_per_shot": round(3600.0 * S / t_seq, 1),
             "speedup": round(t_seq / t_batch, 3),
             "gcell_upd_per_s_batched": round(cells * w.steps * 3 * S / t_batch / 1e9, 2),
@@ -165,6 +184,11 @@ def main():
     ap.add_argument("--fwi-scale", default="small", choices=["none", "tiny", "small", "medium"],
                     help="also run plain FWI against learned-preconditioned FWI (time to equal misfit) at this "
                          "grid scale (fwi3d_amd.experiment)")
+    ap.add_argument("--batch-tuning", default="",
+                    help="fixed batched launch shapes instead of autotuning, e.g. 'fwd=8,2,16,1;rtm=4,2,1,4' "
+                         "(BY, RPT, z-chunks, shots per block)")
+    ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
+                    help="capture the batched passes' time loops in hipGraphs (1) or launch on the stream (0)")
     ap.add_argument("--batch-shots", type=int, default=16,
                     help="also time this many shots of the 4x4 surface survey as one batch against the "
                          "per-shot path (0: skip; 16: the full survey)")
@@ -282,8 +306,9 @@ def main():
     ctx_vp = ctx.get_volume(lib.VOL_MODEL) if not args.no_cpu_baseline else None
 
     batched = None
-    if args.batch_shots > 0 and not args.pass_steps:
-        batched = batched_rtm(w, ctx, args.batch_shots, lib, configs)
+    if args.batch_shots > 0:
+        batched = batched_rtm(w, ctx, args.batch_shots, lib, configs, args.batch_tuning, args.pass_steps,
+                              bool(args.batch_graph))
 
     fwi_res = None
     if args.fwi_scale != "none" and not args.pass_steps:
