@@ -76,6 +76,39 @@ def cpu_baseline(w, vp_pad, seconds):
                       % (nf, nbk, t_fwd + t_bwd)}
 
 
+def cpu_vs_gpu_two_layer(lib, configs):
+    """The reference's own CPU-sized case -- the 101^3 two-layer forward
+    modeling (nb = 50, 1000 samples, 81 x 81 receivers) -- run in full on the
+    GPU and by the oracle on the host: wall times and a bit-for-bit check of
+    the 1000 x 6561 seismogram."""
+    import oracle as O
+    w = configs.two_layer_101()
+    vp = O.random_boundary(O.expand_boundary(w.interior, w.nb), w.nb, w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+    c = lib.Context(w.nxx, w.nyy, w.nzz, w.nb, w.dh, w.dt, w.nt, w.tlag)
+    c.set_model_interior(w.interior)
+    c.random_boundary(w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+    c.set_wavelet(w.wavelet)
+    c.set_source(*w.src)
+    c.set_receivers(w.rec)
+    c.autotune(lib.KIND_FWD_RBC, 4)
+    c.forward(lib.BOUNDARY_RBC)
+    c.sync()
+    t0 = time.perf_counter()
+    c.forward(lib.BOUNDARY_RBC)
+    c.sync()
+    t_gpu = time.perf_counter() - t0
+    g = c.seismogram()
+    c.close()
+    t0 = time.perf_counter()
+    _, _, seis = O.forward(vp, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src, w.rec)
+    t_cpu = time.perf_counter() - t0
+    updates = w.cell_updates(1)
+    return {"workload": "two-layer 101^3 forward modeling, nb=50, 1000 samples (1079 steps), 81x81 receivers",
+            "gpu_s": round(t_gpu, 4), "cpu_s": round(t_cpu, 3), "speedup": round(t_cpu / t_gpu, 1),
+            "gpu_gcell_upd_s": round(updates / t_gpu / 1e9, 2), "cpu_gcell_upd_s": round(updates / t_cpu / 1e9, 3),
+            "seismogram_bit_exact": bool(np.array_equal(g, seis))}
+
+
 def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=True):
     """RTM of the first `nshots` shots of the 4x4 surface survey (each shot its
     own random boundary and its own 201x201 receiver patch clipped at the
@@ -338,6 +371,8 @@ def main():
             log("warning: device model differs from the oracle's")
         ctx.close()
         cpu = cpu_baseline(w, vp_pad, args.cpu_seconds)
+        log("two-layer 101^3 forward modeling: GPU and CPU oracle ...")
+        cpu["two_layer_101_full_run"] = cpu_vs_gpu_two_layer(lib, configs)
     else:
         ctx.close()
 

@@ -34,3 +34,15 @@ def test_create_without_device_fails_cleanly():
         return
     c.close()
     pytest.skip("a GPU is present")
+
+
+def test_create_rejects_volumes_of_2_gib_and_more():
+    """The step kernels address a padded volume with 32-bit byte offsets below
+    2^31: creation refuses larger grids before touching a device."""
+    from fwi3d_amd import lib
+    with pytest.raises(lib.FwiError) as e:
+        lib.Context(1024, 1024, 512, 50, 10.0, 1e-3, 10, 2)  # 1024*1024*512*4 B = 2 GiB
+    assert e.value.code == -5  # FWI_ERR_RANGE
+    with pytest.raises(lib.FwiError) as e:
+        lib.Context(40, 40, 40, 8, 10.0, 1e-3, 10, 2, nshots=0)
+    assert e.value.code == -1  # FWI_ERR_ARG

@@ -103,7 +103,7 @@ def test_batched_rtm_matches_per_shot(fl, sb):
     assert rel_l2(ill_b, illum) <= TOL_BATCH, rel_l2(ill_b, illum)
 
 
-def _interleaved_oracle_rtm(case, shots, vps, datas, sb):
+def _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=False):
     """The batched RTM's summation order on the CPU oracle: every time step
     advances the shots in slot order, each group of sb consecutive shots
     adds its terms into its own image/illumination partial, and the partials
@@ -120,7 +120,7 @@ def _interleaved_oracle_rtm(case, shots, vps, datas, sb):
         for s in range(S):
             gi = s // sb
             out = O.backward(vps[s], case.dh, case.dt, case.nt, datas[s], shots[s][1], *st[s], img[gi], ill[gi],
-                             b_begin=b, b_end=b + 1, tlag=case.tlag)
+                             fwi=fwi, b_begin=b, b_end=b + 1, tlag=case.tlag)
             st[s] = list(out[:4])
             img[gi], ill[gi] = out[4], out[5]
     fimg, fill = img[0].copy(), ill[0].copy()
@@ -164,6 +164,44 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
     assert np.abs(want_img).max() > 0
     assert np.array_equal(ill_b, want_ill)
     assert np.array_equal(img_b, want_img)
+
+
+def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
+    """FWI gradient mode, two shots per block, and a shot with no receivers
+    (its receiver field stays zero, its illumination still counts): bit for
+    bit against the oracle in the batched summation order."""
+    case = Case(nx=40, ny=36, nz=28, nt=50, amp=0.0)
+    S, sb = 4, 2
+    shots = _shots(case, S)
+    shots[1] = (shots[1][0], np.zeros((0, 3), np.int32))
+    padded = O.expand_boundary(case.interior, case.nb)
+    vps = [O.random_boundary(padded, case.nb, 99, s, 0.6, case.vclamp) for s in range(S)]
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    for s, (src, rec) in enumerate(shots):
+        c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+        c.set_source(*src, slot=s)
+        c.set_receivers(rec, slot=s)
+    c.set_wavelet(case.wavelet)
+    for kind in (0, 3):
+        c.set_shots_per_block(kind, sb)
+    c.forward(fl.BOUNDARY_RBC)
+    assert c.seismogram(slot=1).shape == (0, case.nt)
+    rng = np.random.default_rng(7)
+    datas = []
+    for s in range(S):
+        d = rng.normal(0, 1e-2, (len(shots[s][1]), case.nt)).astype(np.float32)
+        datas.append(d)
+        c.set_backward_data(d, slot=s)
+    c.zero_volume(fl.VOL_GRAD)
+    c.grad_backward()
+    g_b = c.get_volume(fl.VOL_GRAD)
+    ill_b = c.get_volume(fl.VOL_ILLUM)
+    c.close()
+    want_g, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=True)
+    assert np.abs(want_g).max() > 0
+    assert np.array_equal(ill_b, want_ill)
+    assert np.array_equal(g_b, want_g)
 
 
 def test_batched_models_match_oracle(fl):

@@ -239,3 +239,30 @@ def test_two_layer_101_cube_seismogram(fl):
         g = c.seismogram()
     assert g.shape == (81 * 81, 1000)
     assert np.array_equal(g, seis), rel_l2(g, seis)
+
+
+def test_smallest_grids_forward_and_rtm(fl):
+    """Grids whose updated region is a few cells wide (no boundary layer;
+    one partial tile per axis): forward and RTM bit for bit against the
+    oracle."""
+    rng = np.random.default_rng(11)
+    for nxx, nyy, nzz in ((12, 11, 13), (9, 9, 9), (70, 10, 10)):
+        vp = rng.uniform(1500, 2500, (nyy, nxx, nzz)).astype(np.float32)
+        wav, tlag = O.wavelet(40, 1e-3, 45.0)
+        src = (nxx // 2, nyy // 2, nzz // 2)
+        X, Y = np.meshgrid(np.arange(4, nxx - 4), np.arange(4, nyy - 4), indexing="ij")
+        rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 4)], axis=1).astype(np.int32)
+        P, Pold, seis = O.forward(vp, 0, 10.0, 1e-3, 40, tlag, wav, src, rec)
+        data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+        want = O.backward(vp, 10.0, 1e-3, 40, data, rec, P, Pold, tlag=tlag)
+        with fl.Context(nxx, nyy, nzz, 0, 10.0, 1e-3, 40, tlag) as c:
+            c.set_model(vp)
+            c.set_wavelet(wav)
+            c.set_source(*src)
+            c.set_receivers(rec)
+            c.forward(fl.BOUNDARY_RBC)
+            assert np.array_equal(c.seismogram(), seis), (nxx, nyy, nzz)
+            c.set_backward_data(data)
+            c.rtm_backward()
+            assert np.array_equal(c.get_volume(fl.VOL_IMAGE), want[4]), (nxx, nyy, nzz)
+            assert np.array_equal(c.get_volume(fl.VOL_ILLUM), want[5]), (nxx, nyy, nzz)
