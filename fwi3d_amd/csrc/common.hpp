@@ -84,7 +84,6 @@ struct StepArgs {
     const float* rec_damp;
     float* seis_row;       // NULL: no recording this step
     const int* rec_start;  // per shot: receivers [rec_start[s], rec_start[s+1])
-    long seis_sstride;     // floats between shots' seismogram rows
 };
 
 // Host-side launchers (kernels_step.hip).

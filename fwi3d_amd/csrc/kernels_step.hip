@@ -18,9 +18,11 @@
 //    queue (Dzz needs no LDS and no re-read);
 //  * the centre plane z is staged with a 4-cell halo in LDS, double
 //    buffered, so one barrier per plane suffices; Dxx and Dyy read LDS;
-//  * HBM traffic per cell and step: read cur, old, vp; write new (16 B).
-//    Halo rows are re-read by neighbouring tiles, which the XCD-aware item
-//    order keeps on the same XCD so they hit its L2.
+//  * HBM traffic per cell and step: read cur, old, vp; write new (16 B),
+//    plus the halo rows a tile re-reads from its neighbours (8 of every TY
+//    rows and 8 of every 64 columns of the stencil field; the XCD-aware
+//    item order keeps neighbouring tiles on one XCD, but most of these
+//    re-reads still miss its L2 at the 401 x 401 x 201 size).
 //  * old and new share one buffer: each cell reads its own old value and
 //    overwrites it, so no second field write is needed.  The ABC damping of
 //    the current field (the reference's P *= damper, which races with the

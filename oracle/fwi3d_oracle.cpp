@@ -93,11 +93,6 @@ inline float damper_at(const float* b1d, const float* b2d, const float* b3d,
     return b3d[di + dj * nb + (long)dk * nb * nb]; // modeling.cu:486-517
 }
 
-template <typename T>
-struct Volume {
-    std::vector<T> v;
-};
-
 }  // namespace
 
 extern "C" {
