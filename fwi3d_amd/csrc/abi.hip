@@ -34,6 +34,7 @@ struct fwi_ctx {
     Layout L{};
     int nshots = 1;
     int arith = FWI_ARITH_REFERENCE;
+    int phys = 0;                        // FWI_PHYS_* bits of the backward passes
     float idh2 = 0, idh3 = 0, dt2 = 0;
 
     float* vp = nullptr;                 // nshots volumes
@@ -322,11 +323,17 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     if (b0 < b1 && b0 < nt)
         HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
                                      c->data_tm + (long)(nt - 1 - b0) * c->nrec_total, c->idh3, c->st));
+    a.phys = c->phys;
+    const int nsteps = nt + c->g.tlag;  // steps of the forward pass being reversed
     for (int b = b0; b < b1; ++b) {
         a.cur = c->f[1 - c->fcur];
         a.old = c->f[c->fcur];
         a.rcur = c->r[c->rcur];
         a.rold = c->r[1 - c->rcur];
+        // the newer source-side field of step b is the forward field of step
+        // nsteps - b, which carries that step's source term
+        const int ns = nsteps - b;
+        a.inj = (c->phys & FWI_PHYS_SUBTRACT_SOURCE) && ns >= 0 && ns < nt ? c->idh3 * c->wavelet[ns] : 0.0f;
         CHKRC(launch_step_timed(c, mode, t, a));
         c->fcur ^= 1;
         c->rcur ^= 1;
@@ -861,6 +868,14 @@ int fwi_set_tuning(fwi_ctx* c, int kind, int by, int rpt, int zchunks) {
     if (zchunks < 1 || zchunks > (c->g.nzz - 8) || !fwi::step_config_supported(kind, c->arith, t))
         return fail(c, FWI_ERR_ARG, "unsupported tile configuration");
     c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_set_physics(fwi_ctx* c, int flags) {
+    if (!c) return FWI_ERR_ARG;
+    if (flags & ~(FWI_PHYS_GRAD_DYY | FWI_PHYS_SUBTRACT_SOURCE | FWI_PHYS_ALIGNED_IMAGING))
+        return fail(c, FWI_ERR_ARG, "unknown physics flag");
+    c->phys = flags;
     return FWI_OK;
 }
 

@@ -30,6 +30,14 @@ constexpr double CFL_MAX = 0.452856;
 enum Mode { FWD_RBC = 0, FWD_ABC = 1, BWD_RTM = 2, BWD_FWI = 3 };
 enum Arith { ARITH_REF = 0, ARITH_FP32 = 1 };
 
+// Optional corrections of the backward pass (bits of StepArgs::phys; the
+// default 0 is the reference's behaviour).
+enum Physics {
+    PHYS_GRAD_DYY = 1,         // FWI gradient with d2/dy2 (inversion.cu:466 omits it)
+    PHYS_SUBTRACT_SOURCE = 2,  // remove the injected source during reconstruction
+    PHYS_ALIGNED_IMAGING = 4,  // image/illumination from the source field at the receiver field's time
+};
+
 struct Layout {
     int nx, ny, nz;  // padded sizes (x = reference nxx, y = nyy, z = nzz)
     int px;          // row pitch in floats
@@ -73,9 +81,12 @@ struct StepArgs {
     float* illum;          // illumination (one per shot group)
     long istride;          // floats between shot groups' image/illum
     float idh2, dt2, dt;
-    // forward source injection of the next step (owner thread adds inj)
-    const long* src_off;   // per shot, device offset of the source cell (forward only)
-    float inj;             // idh3 * wavelet[n+1], 0 when n+1 >= nt
+    // source term: forward, added to the next step's field by the thread
+    // that writes the source cell; backward with PHYS_SUBTRACT_SOURCE, added
+    // back to the reconstructed field (the term the newer field carries)
+    const long* src_off;   // per shot, device offset of the source cell
+    float inj;             // forward: idh3 * wavelet[n+1] (0 when n+1 >= nt)
+    int phys;              // backward: Physics bits
     // ABC
     const float* b1d;
     int nb;

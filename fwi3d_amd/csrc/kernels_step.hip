@@ -9,6 +9,9 @@
 //       S   = dt^2 vp^2 L(So) + 2 So - S   (time-reversed source field)
 //       Ro  = dt^2 vp^2 L(R)  + 2 R  - Ro  (receiver field)
 //       illum += S^2;  image += S R   |  grad += dt R (Dxx + Dzz)(So) vp^2
+//   with the optional backward corrections of StepArgs::phys: the source
+//   term put back into S during reconstruction, So instead of S in the
+//   imaging condition, Dyy in the gradient.
 //
 // Work decomposition (gfx950, wave64):
 //  * a block owns a 64 x TY tile of the x-y plane (TY = BY*RPT rows; each of
@@ -248,7 +251,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             rb = make_rsrc(a.rold + sbase, vol_b);
         }
         int src_z = -1, src_c = -1;
-        if constexpr (!BWD) {
+        if (!BWD || (a.phys & PHYS_SUBTRACT_SOURCE)) {  // uniform
             const long so = a.src_off[shot];
             if (so >= 0) {
                 src_z = (int)(so / L.pp);
@@ -415,18 +418,22 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     float sn = K * ((dxx + dyy) + dzz);
                     sn = sn + 2.0f * c;
                     sn = sn - o[r];
+                    if ((a.phys & PHYS_SUBTRACT_SOURCE) && z == src_z && cell[r] == src_c) sn = sn + a.inj;
                     float rn = K * ((rxx + ryy) + rzz);
                     rn = rn + 2.0f * cr;
                     rn = rn - ro[r];
                     res[0][r] = sn;
                     res[1][r] = rn;
-                    const float t_ill = sn * sn;
+                    // source field paired with R: the reconstructed one (one
+                    // step behind R, the reference) or the current one
+                    const float si = (a.phys & PHYS_ALIGNED_IMAGING) ? c : sn;
+                    const float t_ill = si * si;
                     float t_img;
                     if constexpr (MODE == BWD_RTM) {
-                        t_img = sn * cr;
+                        t_img = si * cr;
                     } else {
                         float gterm = a.dt * cr;
-                        gterm = gterm * (dxx + dzz);
+                        gterm = gterm * ((a.phys & PHYS_GRAD_DYY) ? (dxx + dyy) + dzz : dxx + dzz);
                         t_img = gterm * vp2;
                     }
                     if (owner) {

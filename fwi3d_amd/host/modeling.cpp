@@ -58,6 +58,22 @@ void Modeling::set_build_parameters() {
     if (a == "reference") arith = FWI_ARITH_REFERENCE;
     else if (a == "fp32") arith = FWI_ARITH_FP32;
     else throw std::invalid_argument("arithmetic must be 'reference' or 'fp32'");
+    physics = 0;
+    std::string corr = catch_parameter_or("backward_corrections", parameters, "none");
+    size_t p0 = 0;
+    while (p0 <= corr.size()) {
+        size_t p1 = corr.find(',', p0);
+        if (p1 == std::string::npos) p1 = corr.size();
+        std::string tok = corr.substr(p0, p1 - p0);
+        tok.erase(0, tok.find_first_not_of(" \t"));
+        tok.erase(tok.find_last_not_of(" \t") + 1);
+        if (tok == "gradient_dyy") physics |= FWI_PHYS_GRAD_DYY;
+        else if (tok == "subtract_source") physics |= FWI_PHYS_SUBTRACT_SOURCE;
+        else if (tok == "aligned_imaging") physics |= FWI_PHYS_ALIGNED_IMAGING;
+        else if (!(tok.empty() || tok == "none"))
+            throw std::invalid_argument("backward_corrections: unknown item '" + tok + "'");
+        p0 = p1 + 1;
+    }
 }
 
 // modeling.cu:36-59: Ricker wavelet with peak frequency fmax/3.
@@ -118,6 +134,7 @@ void Modeling::create_context(int boundary_cells, fwi_ctx** out) {
     if (rc != FWI_OK) throw std::runtime_error(std::string("fwi_create_batch: ") + fwi_last_error(nullptr));
     fwi_ctx* c = *out;
     check(fwi_set_arith(c, arith), "fwi_set_arith");
+    check(fwi_set_physics(c, physics), "fwi_set_physics");
     check(fwi_set_model_interior(c, model.data()), "fwi_set_model_interior");
     check(fwi_set_wavelet(c, wavelet.data()), "fwi_set_wavelet");
 }

@@ -15,6 +15,9 @@
 //   rbc_amplitude = A   random boundary amplitude, +-A*v at the outer edge
 //                       (default 0.6)
 //   arithmetic    = reference | fp32   (default reference)
+//   backward_corrections = comma-separated subset of
+//                   gradient_dyy, subtract_source, aligned_imaging
+//                       (default none: the reference's backward passes)
 #pragma once
 
 #include <cstdint>
@@ -54,6 +57,7 @@ protected:
     uint64_t rbc_seed = 20261203ull;
     float rbc_amplitude = 0.6f;
     int arith = FWI_ARITH_REFERENCE;
+    int physics = 0;  // FWI_PHYS_* bits for the backward passes
 
     void set_wavelet();
     void set_geometry();

@@ -21,12 +21,13 @@ ERR = {-1: "FWI_ERR_ARG", -2: "FWI_ERR_HIP", -3: "FWI_ERR_CFL", -4: "FWI_ERR_STA
        -5: "FWI_ERR_RANGE", -6: "FWI_ERR_NONFINITE", -7: "FWI_ERR_NODEV"}
 BOUNDARY_RBC, BOUNDARY_ABC = 0, 1
 ARITH_REFERENCE, ARITH_FP32 = 0, 1
+PHYS_GRAD_DYY, PHYS_SUBTRACT_SOURCE, PHYS_ALIGNED_IMAGING = 1, 2, 4
 VOL_MODEL, VOL_FIELD, VOL_FIELD_OLD, VOL_RECV, VOL_RECV_OLD, VOL_IMAGE, VOL_ILLUM, VOL_GRAD = range(8)
 KIND_FWD_RBC, KIND_FWD_ABC, KIND_BWD_RTM, KIND_BWD_FWI = range(4)
 
 # every symbol declared in include/fwi3d.h
 EXPORTS = [
-    "fwi_create", "fwi_destroy", "fwi_last_error", "fwi_set_arith", "fwi_set_model", "fwi_set_model_interior",
+    "fwi_create", "fwi_destroy", "fwi_last_error", "fwi_set_arith", "fwi_set_physics", "fwi_set_model", "fwi_set_model_interior",
     "fwi_random_boundary", "fwi_set_wavelet", "fwi_set_dampers", "fwi_set_source", "fwi_set_receivers",
     "fwi_forward", "fwi_forward_steps", "fwi_get_seismogram", "fwi_set_backward_data", "fwi_rtm_backward",
     "fwi_grad_backward", "fwi_backward_steps", "fwi_zero_volume", "fwi_get_volume", "fwi_set_volume",
@@ -69,6 +70,7 @@ def load():
         "fwi_create": [ctypes.POINTER(Grid), I, ctypes.POINTER(P)],
         "fwi_destroy": [P],
         "fwi_set_arith": [P, I],
+        "fwi_set_physics": [P, I],
         "fwi_set_model": [P, f32],
         "fwi_set_model_interior": [P, f32],
         "fwi_random_boundary": [P, ctypes.c_uint64, I, F, F],
@@ -168,6 +170,10 @@ class Context:
     # -- set-up ------------------------------------------------------------
     def set_arith(self, arith):
         self._chk(self.L.fwi_set_arith(self.h, arith))
+
+    def set_physics(self, flags):
+        """PHYS_* bits of the backward passes (0 = the reference's behaviour)."""
+        self._chk(self.L.fwi_set_physics(self.h, flags))
 
     def set_model(self, vp_pad):
         self._chk(self.L.fwi_set_model(self.h, self._vol(vp_pad)))

@@ -83,6 +83,17 @@ const char* fwi_last_error(const fwi_ctx* ctx);
 /* FWI_ARITH_REFERENCE (default) or FWI_ARITH_FP32. */
 int fwi_set_arith(fwi_ctx* ctx, int arith);
 
+/* Optional corrections of the backward passes (bit mask; 0, the default,
+ * reproduces the reference).  No reference counterpart: the three flags
+ * switch off the reference behaviours of inversion.cu:466 (gradient without
+ * d2/dy2), migration.cu:195 / inversion.cu:461 (reconstruction keeps the
+ * injected source) and migration.cu:195-200 (the reconstructed source field
+ * is paired with the receiver field one step later). */
+#define FWI_PHYS_GRAD_DYY 1          /* FWI gradient with d2/dy2 */
+#define FWI_PHYS_SUBTRACT_SOURCE 2   /* remove the source term during reconstruction */
+#define FWI_PHYS_ALIGNED_IMAGING 4   /* image/illumination from the source field at the receiver field's time */
+int fwi_set_physics(fwi_ctx* ctx, int flags);
+
 /* Padded velocity model, host layout (cudaMemcpy of d_Vp, modeling.cu:176).
  * Rejects NaN/Inf and CFL violations. */
 int fwi_set_model(fwi_ctx* ctx, const float* vp_padded);

@@ -322,10 +322,22 @@ void forward_impl(const float* vp, int nxx, int nyy, int nzz, int nb, float dh, 
 // gradient (build_gradient, inversion.cu:400-468), with the receiver
 // injection of inject_seismogram / inject_residuo (migration.cu:119-132,
 // inversion.cu:385-398) applied in receiver order before the step.
+//
+// phys (0 = the reference) switches on corrections the reference lacks:
+//   1: FWI gradient with d2/dy2 (inversion.cu:466 omits it);
+//   2: source subtraction -- the newer source-side field of backward step b
+//      is the forward field of step nsteps - b, which carries the source term
+//      idh3 * wavelet[nsteps - b] (for nsteps - b < nt); it is added back to
+//      the reconstructed field at the source cell, which makes the
+//      reconstruction the exact reverse of the forward recurrence;
+//   4: aligned imaging -- the image and illumination use the current source
+//      field So (the time of the receiver field R) instead of the
+//      reconstructed one.
 template <typename T>
 void backward_impl(const float* vp, int nxx, int nyy, int nzz, float dh, float dt, int nt, const float* data,
                    int nrec, const int* rIdx, const int* rIdy, const int* rIdz, int fwi, int b_begin, int b_end,
-                   T* S, T* So, T* R, T* Ro, T* image, T* illum) {
+                   T* S, T* So, T* R, T* Ro, T* image, T* illum, int phys = 0, const float* wavelet = nullptr,
+                   long src = -1, int nsteps = 0) {
     Dims d{nxx, nyy, nzz};
     const float idh2f = 1.0f / (dh * dh);
     const float idh3f = 1.0f / (dh * dh * dh);
@@ -339,6 +351,9 @@ void backward_impl(const float* vp, int nxx, int nyy, int nzz, float dh, float d
         if (b < nt)
             for (int r = 0; r < nrec; r++)
                 Pr[lidx(rIdz[r], rIdx[r], rIdy[r], nxx, nzz)] += idh3 * (T)data[(long)(nt - b - 1) + (long)r * nt];
+        const int ns = nsteps - b;
+        const bool sub = (phys & 2) && src >= 0 && ns >= 0 && ns < nt;
+        const T src_term = sub ? idh3 * (T)wavelet[ns] : (T)0;
 #pragma omp parallel for schedule(static)
         for (int k = 4; k < nyy - 4; k++)
             for (int j = 4; j < nxx - 4; j++) {
@@ -353,13 +368,16 @@ void backward_impl(const float* vp, int nxx, int nyy, int nzz, float dh, float d
                     T rxx = Deriv<T>::d2(Pr, c, sx, idh2);
                     T ryy = Deriv<T>::d2(Pr, c, sy, idh2);
                     T rzz = Deriv<T>::d2(Pr, c, 1, idh2);
-                    Ps[c] = tdt * tdt * vp2 * (sxx + syy + szz) + (T)2 * Psold[c] - Ps[c];
+                    T sn = tdt * tdt * vp2 * (sxx + syy + szz) + (T)2 * Psold[c] - Ps[c];
+                    if (sub && c == src) sn = sn + src_term;
+                    Ps[c] = sn;
                     Prold[c] = tdt * tdt * vp2 * (rxx + ryy + rzz) + (T)2 * Pr[c] - Prold[c];
-                    illum[c] += Ps[c] * Ps[c];
+                    const T si = (phys & 4) ? Psold[c] : Ps[c];
+                    illum[c] += si * si;
                     if (fwi)
-                        image[c] += tdt * Pr[c] * (sxx + szz) * vp2;  // inversion.cu:466 (no d2/dy2)
+                        image[c] += tdt * Pr[c] * ((phys & 1) ? sxx + syy + szz : sxx + szz) * vp2;  // inversion.cu:466
                     else
-                        image[c] += Ps[c] * Pr[c];                    // migration.cu:200
+                        image[c] += si * Pr[c];  // migration.cu:200
                 }
             }
         std::swap(Ps, Psold);
@@ -422,6 +440,26 @@ void oracle_backward_f64(const float* vp, int nxx, int nyy, int nzz, float dh, f
                          double* illum) {
     backward_impl<double>(vp, nxx, nyy, nzz, dh, dt, nt, data, nrec, rIdx, rIdy, rIdz, fwi, b_begin, b_end, S, So,
                           R, Ro, image, illum);
+}
+
+// As oracle_backward_f32/f64 with the corrections `phys` (see backward_impl);
+// wavelet (nt samples), the source cell (sIdx, sIdy, sIdz) and the forward
+// pass length nsteps = nt + tlag serve the source subtraction.
+void oracle_backward_phys_f32(const float* vp, int nxx, int nyy, int nzz, float dh, float dt, int nt,
+                              const float* data, int nrec, const int* rIdx, const int* rIdy, const int* rIdz, int fwi,
+                              int b_begin, int b_end, float* S, float* So, float* R, float* Ro, float* image,
+                              float* illum, int phys, const float* wavelet, int sIdx, int sIdy, int sIdz, int nsteps) {
+    backward_impl<float>(vp, nxx, nyy, nzz, dh, dt, nt, data, nrec, rIdx, rIdy, rIdz, fwi, b_begin, b_end, S, So, R,
+                         Ro, image, illum, phys, wavelet, lidx(sIdz, sIdx, sIdy, nxx, nzz), nsteps);
+}
+
+void oracle_backward_phys_f64(const float* vp, int nxx, int nyy, int nzz, float dh, float dt, int nt,
+                              const float* data, int nrec, const int* rIdx, const int* rIdy, const int* rIdz, int fwi,
+                              int b_begin, int b_end, double* S, double* So, double* R, double* Ro, double* image,
+                              double* illum, int phys, const float* wavelet, int sIdx, int sIdy, int sIdz,
+                              int nsteps) {
+    backward_impl<double>(vp, nxx, nyy, nzz, dh, dt, nt, data, nrec, rIdx, rIdy, rIdz, fwi, b_begin, b_end, S, So,
+                          R, Ro, image, illum, phys, wavelet, lidx(sIdz, sIdx, sIdy, nxx, nzz), nsteps);
 }
 
 // ---------------------------------------------------------------------------

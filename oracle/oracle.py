@@ -56,6 +56,9 @@ def lib():
            _c_int, _c_int, _c_int]
     L.oracle_backward_f32.argtypes = bwd + [_f32p] * 6
     L.oracle_backward_f64.argtypes = bwd + [_f64p] * 6
+    phys = [_c_int, _f32p, _c_int, _c_int, _c_int, _c_int]
+    L.oracle_backward_phys_f32.argtypes = bwd + [_f32p] * 6 + phys
+    L.oracle_backward_phys_f64.argtypes = bwd + [_f64p] * 6 + phys
     L.oracle_export_seismic.argtypes = [_f32p, _f32p] + [_c_int] * 4 + [_c_float, _f32p]
     L.oracle_gradient_post.argtypes = [_f32p, _f32p] + [_c_int] * 4 + [_c_float, _c_float, _f32p]
     L.oracle_gradient_post.restype = _c_float
@@ -174,9 +177,11 @@ def forward(vp, nb, dh, dt, nt, tlag, wavelet_, src, rec, abc_tables=None, n_beg
 
 
 def backward(vp, dh, dt, nt, data, rec, S, So, R=None, Ro=None, image=None, illum=None, fwi=False,
-             b_begin=0, b_end=None, tlag=None, fp64=False):
+             b_begin=0, b_end=None, tlag=None, fp64=False, phys=0, wavelet_=None, src=None):
     """Steps [b_begin, b_end) of the RTM (migration.cu:61-81) or FWI
     (inversion.cu:235-253) backward pass.  data has shape (nrec, nt).
+    phys: correction bits (1 gradient with d2/dy2, 2 source subtraction --
+    needs wavelet_ and src -- 4 aligned imaging); 0 is the reference.
     Returns (S, So, R, Ro, image, illum)."""
     vp = np.ascontiguousarray(vp, np.float32)
     nyy, nxx, nzz = vp.shape
@@ -189,9 +194,18 @@ def backward(vp, dh, dt, nt, data, rec, S, So, R=None, Ro=None, image=None, illu
 
     S, So, R, Ro, image, illum = arr(S), arr(So), arr(R), arr(Ro), arr(image), arr(illum)
     rx, ry, rz, nrec = _rec_arrays(rec)
-    fn = lib().oracle_backward_f64 if fp64 else lib().oracle_backward_f32
-    fn(vp, nxx, nyy, nzz, dh, dt, nt, np.ascontiguousarray(data, np.float32), nrec, rx, ry, rz,
-       1 if fwi else 0, b_begin, b_end, S, So, R, Ro, image, illum)
+    args = (vp, nxx, nyy, nzz, dh, dt, nt, np.ascontiguousarray(data, np.float32), nrec, rx, ry, rz,
+            1 if fwi else 0, b_begin, b_end, S, So, R, Ro, image, illum)
+    if phys:
+        if (phys & 2) and (wavelet_ is None or src is None or tlag is None):
+            raise ValueError("source subtraction needs wavelet_, src and tlag")
+        w = np.ascontiguousarray(wavelet_ if wavelet_ is not None else np.zeros(nt), np.float32)
+        sx, sy, sz = src if src is not None else (-1, -1, -1)
+        fn = lib().oracle_backward_phys_f64 if fp64 else lib().oracle_backward_phys_f32
+        fn(*args, int(phys), w, int(sx), int(sy), int(sz), int(nt + (tlag or 0)))
+    else:
+        fn = lib().oracle_backward_f64 if fp64 else lib().oracle_backward_f32
+        fn(*args)
     return S, So, R, Ro, image, illum
 
 

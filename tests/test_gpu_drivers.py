@@ -87,3 +87,29 @@ def test_inversion_main_lowers_misfit(tiny, tmp_path):
                                      % (w.nz, w.nx, w.ny)), np.float32).reshape(w.interior.shape)
     assert np.isfinite(final).all()
     assert rel_l2(final, w.interior) < rel_l2(init, w.interior)
+
+
+def test_migration_main_backward_corrections(tiny, tmp_path):
+    """The backward_corrections key reaches the library: with source
+    subtraction and aligned imaging the RTM section changes (and stays
+    finite); an unknown item is refused."""
+    w = tiny
+    imgs = []
+    for name, corr in (("ref", None), ("corr", "subtract_source, aligned_imaging")):
+        folder = str(tmp_path / name)
+        params = w.write(folder, batch_shots=3)
+        _run("modeling_main", params)
+        if corr:
+            with open(params, "a") as f:
+                f.write("backward_corrections = %s\n" % corr)
+        _run("migration_main", params)
+        f = os.path.join(folder, "outputs/seismic", "RTM_section_45Hz_%dx%dx%d_10m.bin" % (w.nz, w.nx, w.ny))
+        imgs.append(np.fromfile(f, np.float32))
+    assert np.isfinite(imgs[1]).all() and np.abs(imgs[1]).max() > 0
+    assert rel_l2(imgs[1], imgs[0]) > 1e-3
+    lines = [l for l in open(params) if not l.startswith("backward_corrections")]
+    with open(params, "w") as f:
+        f.writelines(lines + ["backward_corrections = gradient_dyy, bogus\n"])
+    r = subprocess.run([os.path.join(BIN, "migration_main"), params], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=240)
+    assert r.returncode != 0 and "bogus" in r.stdout

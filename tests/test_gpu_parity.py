@@ -89,6 +89,28 @@ def test_backward_bit_exact(fl, fwi):
     assert np.array_equal(g_Ro, Ro)
 
 
+@pytest.mark.parametrize("phys", [1, 2, 4, 7])
+@pytest.mark.parametrize("fwi", [False, True])
+def test_backward_corrections_bit_exact(fl, fwi, phys):
+    """The optional backward corrections (gradient with d2/dy2, source
+    subtraction, aligned imaging, all three) against the oracle's."""
+    case = Case(nt=70)
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(phys)
+    data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+    S, So, R, Ro, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=fwi,
+                                            tlag=case.tlag, phys=phys, wavelet_=case.wavelet, src=case.src)
+    with case.context(fl.Context) as c:
+        c.set_physics(phys)
+        c.forward(fl.BOUNDARY_RBC)
+        c.set_backward_data(data)
+        (c.grad_backward if fwi else c.rtm_backward)()
+        assert np.array_equal(c.get_volume(fl.VOL_ILLUM), illum)
+        assert np.array_equal(c.get_volume(fl.VOL_GRAD if fwi else fl.VOL_IMAGE), image)
+        assert np.array_equal(c.get_volume(fl.VOL_FIELD), S)
+        assert np.array_equal(c.get_volume(fl.VOL_RECV), R)
+
+
 def test_step_ranges_compose(fl):
     """forward_steps(0,k) + forward_steps(k,N) == forward_steps(0,N) == oracle."""
     case = Case(nt=60)

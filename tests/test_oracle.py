@@ -15,6 +15,11 @@ function is checked against what the mathematics fixes:
                              gives a zero image; image/illum linear/quadratic in
                              scale; FWI gradient term vanishes for a field
                              without x/z curvature
+  backward corrections       source subtraction makes the reconstruction exact
+                             through the source window; the gradient with d2/dy2
+                             is x<->y symmetric for a symmetric experiment;
+                             aligned imaging equals a step-by-step replay of
+                             So * R and So^2
   wavelet                    Ricker closed form (FP64) and tlag = 79 / 74
   dampers                    b1 = exp(-(f(nb-i))^2); b2 = b1a+b1b-1; b3 ~ 2*sum-5;
                              lookup symmetric under face mirroring
@@ -122,6 +127,81 @@ def test_reconstruction_recovers_forward():
     S, So, *_ = O.backward(vp, dh, dt, nt, data, rec, P, Pold, b_begin=0, b_end=m, tlag=tlag, fp64=True)
     # (S, So) starts at (u_N, u_{N-1}) and steps to (u_{N-b}, u_{N-1-b}); P_m = u_{N-m}
     assert np.abs(S - P_m).max() <= 1e-9 * np.abs(P_m).max()
+
+
+def test_source_subtraction_makes_reconstruction_exact():
+    """With the source term put back (correction bit 2) the reconstruction
+    reverses the forward recurrence through the source window too: So after
+    m backward steps is the forward's previous field of step N-1-m.  Without
+    it (the reference) the fields near the source drift."""
+    rng = np.random.default_rng(2)
+    vp = (2200 + rng.uniform(-300, 300, (36, 40, 32))).astype(np.float32)
+    nt, dt, dh = 60, 1e-3, 10.0
+    w, tlag = O.wavelet(nt, dt, 45.0)
+    N = nt + tlag
+    src, rec = (20, 18, 16), np.array([[10, 10, 8]], np.int32)
+    m = N - 25  # back to step 24, inside the source window
+    _, Pold_m, _ = O.forward(vp, 6, dh, dt, nt, tlag, w, src, rec, n_end=N - m, fp64=True)
+    P, Pold, _ = O.forward(vp, 6, dh, dt, nt, tlag, w, src, rec, fp64=True)
+    data = np.zeros((1, nt), np.float32)
+    kw = dict(b_begin=0, b_end=m, tlag=tlag, fp64=True)
+    _, So, *_ = O.backward(vp, dh, dt, nt, data, rec, P, Pold, phys=2, wavelet_=w, src=src, **kw)
+    scale = np.abs(Pold_m).max()
+    assert np.abs(So - Pold_m).max() <= 1e-9 * scale
+    _, So_ref, *_ = O.backward(vp, dh, dt, nt, data, rec, P, Pold, **kw)
+    assert np.abs(So_ref - Pold_m).max() > 1e-3 * scale
+
+
+def test_gradient_with_dyy_is_symmetric_in_x_and_y():
+    """An experiment symmetric under x <-> y (layered model, source on the
+    diagonal, square receiver grid) gives, with d2/dy2 in the gradient
+    (correction bit 1), a gradient symmetric under x <-> y bit for bit; the
+    reference gradient (Dxx + Dzz only) is not symmetric."""
+    n, nz = 34, 30
+    z = np.arange(nz)
+    vp = np.broadcast_to(np.where(z < 15, 1800.0, 2400.0).astype(np.float32), (n, n, nz)).copy()
+    nt, dt, dh = 50, 1e-3, 10.0
+    w, tlag = O.wavelet(nt, dt, 45.0)
+    g = np.arange(9, 26, 4)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 6)], axis=1).astype(np.int32)
+    P, Pold, seis = O.forward(vp, 6, dh, dt, nt, tlag, w, (17, 17, 6), rec)
+    g1 = O.backward(vp, dh, dt, nt, seis, rec, P, Pold, fwi=True, tlag=tlag, phys=1)[4]
+    g0 = O.backward(vp, dh, dt, nt, seis, rec, P, Pold, fwi=True, tlag=tlag)[4]
+    assert np.abs(g1).max() > 0
+    assert np.array_equal(g1, g1.transpose(1, 0, 2))
+    assert not np.array_equal(g0, g0.transpose(1, 0, 2))
+
+
+def test_aligned_imaging_pairs_the_current_source_field():
+    """With aligned imaging (correction bit 4) every step adds So * R and So^2,
+    So being the source field the step reads and R the receiver field after
+    the step's injection: replay the steps one at a time and accumulate those
+    products in numpy (float32, the same order)."""
+    rng = np.random.default_rng(8)
+    shape = (30, 32, 28)
+    vp = (2000 + rng.uniform(-100, 100, shape)).astype(np.float32)
+    nt, dt, dh = 40, 1e-3, 10.0
+    w, tlag = O.wavelet(nt, dt, 45.0)
+    rec = np.array([[12, 12, 8], [18, 16, 8], [12, 12, 8]], np.int32)
+    P, Pold, seis = O.forward(vp, 6, dh, dt, nt, tlag, w, (15, 15, 10), rec)
+    data = rng.normal(0, 1, seis.shape).astype(np.float32)
+    idh3 = np.float32(1.0) / (np.float32(dh) * np.float32(dh) * np.float32(dh))
+    inner = (slice(4, -4),) * 3
+    state, img, ill = [P, Pold, np.zeros(shape, np.float32), np.zeros(shape, np.float32)], None, None
+    want_img, want_ill = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+    for b in range(30):
+        c = state[1].copy()
+        R = state[2].copy()
+        for r, (x, y, zz) in enumerate(rec):
+            R[y, x, zz] = R[y, x, zz] + idh3 * data[r, nt - 1 - b]
+        want_img[inner] = want_img[inner] + c[inner] * R[inner]
+        want_ill[inner] = want_ill[inner] + c[inner] * c[inner]
+        out = O.backward(vp, dh, dt, nt, data, rec, *state, img, ill, b_begin=b, b_end=b + 1, tlag=tlag, phys=4)
+        state, img, ill = list(out[:4]), out[4], out[5]
+    assert np.abs(want_img).max() > 0
+    assert np.array_equal(img, want_img)
+    assert np.array_equal(ill, want_ill)
 
 
 def test_receiver_field_equals_forward_of_reversed_data():
