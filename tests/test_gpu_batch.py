@@ -103,7 +103,7 @@ def test_batched_rtm_matches_per_shot(fl, sb):
     assert rel_l2(ill_b, illum) <= TOL_BATCH, rel_l2(ill_b, illum)
 
 
-def _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=False):
+def _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=False, phys=0):
     """The batched RTM's summation order on the CPU oracle: every time step
     advances the shots in slot order, each group of sb consecutive shots
     adds its terms into its own image/illumination partial, and the partials
@@ -120,7 +120,8 @@ def _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=False):
         for s in range(S):
             gi = s // sb
             out = O.backward(vps[s], case.dh, case.dt, case.nt, datas[s], shots[s][1], *st[s], img[gi], ill[gi],
-                             fwi=fwi, b_begin=b, b_end=b + 1, tlag=case.tlag)
+                             fwi=fwi, b_begin=b, b_end=b + 1, tlag=case.tlag, phys=phys,
+                             wavelet_=case.wavelet, src=shots[s][0])
             st[s] = list(out[:4])
             img[gi], ill[gi] = out[4], out[5]
     fimg, fill = img[0].copy(), ill[0].copy()
@@ -167,9 +168,10 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
 
 
 def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
-    """FWI gradient mode, two shots per block, and a shot with no receivers
-    (its receiver field stays zero, its illumination still counts): bit for
-    bit against the oracle in the batched summation order."""
+    """FWI gradient mode with all backward corrections on, two shots per
+    block, and a shot with no receivers (its receiver field stays zero, its
+    illumination still counts): bit for bit against the oracle in the batched
+    summation order."""
     case = Case(nx=40, ny=36, nz=28, nt=50, amp=0.0)
     S, sb = 4, 2
     shots = _shots(case, S)
@@ -185,6 +187,7 @@ def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
     c.set_wavelet(case.wavelet)
     for kind in (0, 3):
         c.set_shots_per_block(kind, sb)
+    c.set_physics(fl.PHYS_GRAD_DYY | fl.PHYS_SUBTRACT_SOURCE | fl.PHYS_ALIGNED_IMAGING)
     c.forward(fl.BOUNDARY_RBC)
     assert c.seismogram(slot=1).shape == (0, case.nt)
     rng = np.random.default_rng(7)
@@ -198,7 +201,7 @@ def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
     g_b = c.get_volume(fl.VOL_GRAD)
     ill_b = c.get_volume(fl.VOL_ILLUM)
     c.close()
-    want_g, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=True)
+    want_g, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=True, phys=7)
     assert np.abs(want_g).max() > 0
     assert np.array_equal(ill_b, want_ill)
     assert np.array_equal(g_b, want_g)
