@@ -34,8 +34,9 @@ HIP_SOURCES = ["abi.hip", "kernels_step.hip", "kernels_aux.hip", "unet_kernels.h
 CXX = shutil.which("g++") or "g++"
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-ffp-contract=off",
               "-march=x86-64-v3"]
-HOST_SOURCES = ["admin.cpp", "geometry.cpp", "modeling.cpp", "migration.cpp", "inversion.cpp", "capi.cpp"]
-DRIVERS = ["modeling_main", "migration_main", "inversion_main"]
+HOST_SOURCES = ["admin.cpp", "geometry.cpp", "modeling.cpp", "migration.cpp", "inversion.cpp", "dataset.cpp",
+                "capi.cpp"]
+DRIVERS = ["modeling_main", "migration_main", "inversion_main", "setup_main", "preprocess_main"]
 
 
 def _run(cmd, cwd=None):

@@ -1,12 +1,13 @@
 // capi.cpp -- C entry points into the host library for tests: the host-side
 // set-up arithmetic (wavelet, dampers, parameter parsing) that feeds the
-// device library.
+// device library, and the data-preparation steps of dataset.hpp.
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "admin.hpp"
+#include "dataset.hpp"
 
 extern "C" {
 
@@ -37,6 +38,20 @@ int fwi3d_host_catch_parameter(const char* target, const char* file, char* out, 
     } catch (...) {
         return -1;
     }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// dataset.hpp entry points (generate_setup.py's smoothing,
+// generate_input_data.py's normalisation and mute)
+void fwi3d_host_gaussian_filter3d(float* v, int n0, int n1, int n2, double sigma) {
+    gaussian_filter3d(v, n0, n1, n2, sigma);
+}
+
+void fwi3d_host_rtm_input(float* data, int nt, int nr, float dt, float sx, float sy, const float* rx, const float* ry) {
+    rtm_input(data, nt, nr, dt, sx, sy, rx, ry);
 }
 
 }  // extern "C"
