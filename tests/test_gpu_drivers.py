@@ -113,3 +113,43 @@ def test_migration_main_backward_corrections(tiny, tmp_path):
     r = subprocess.run([os.path.join(BIN, "migration_main"), params], stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True, timeout=240)
     assert r.returncode != 0 and "bogus" in r.stdout
+
+
+def test_full_workflow_in_cpp(tiny, tmp_path):
+    """The reference workflow end to end with no Python step: setup_main
+    (acquisition + smoothed initial model), modeling_main, preprocess_main
+    (RTM input data), migration_main on the preprocessed data, and
+    inversion_main from the initial model (the reference's model_file edited
+    to it by hand; here by rewriting that line)."""
+    w = tiny
+    folder = str(tmp_path / "flow")
+    params = w.write(folder, depth_mask=20.0, max_iteration=2, batch_shots=2)
+    text = open(params).read()
+    text = text.replace("mig_input_folder = %s/inputs/data/" % folder,
+                        "mig_input_folder = %s/inputs/data/input_RTM_" % folder)
+    # setup_main writes an 8 x 8 receiver grid: the data file names carry nr64
+    text = text.replace("_nr%d_" % len(w.receivers[0]), "_nr64_")
+    text += ("setup_shots = 2,2\nsetup_shot_range = 100,300,100,250\nsetup_shot_depth = 20\n"
+             "setup_receivers = 8,8\nsetup_receiver_range = 20,370,20,330\nsetup_receiver_depth = 20\n"
+             "setup_smoothing = 2\n")
+    with open(params, "w") as f:
+        f.write(text)
+    _run_plain = lambda exe: subprocess.run([os.path.join(BIN, exe), params], stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True, timeout=240)
+    r = _run_plain("setup_main")
+    assert r.returncode == 0, r.stdout
+    _run("modeling_main", params)
+    r = _run_plain("preprocess_main")
+    assert r.returncode == 0 and "4 shots" in r.stdout, r.stdout
+    data = os.path.join(folder, "inputs/data")
+    assert len([f for f in os.listdir(data) if f.startswith("input_RTM_")]) == 4
+    _run("migration_main", params)
+    img = np.fromfile(os.path.join(folder, "outputs/seismic", "RTM_section_45Hz_%dx%dx%d_10m.bin"
+                                   % (w.nz, w.nx, w.ny)), np.float32)
+    assert img.size == w.nx * w.ny * w.nz and np.isfinite(img).all() and np.abs(img).max() > 0
+    model_true = [l for l in text.splitlines() if l.startswith("model_file")][0].split("=")[1].strip()
+    with open(params, "w") as f:
+        f.write(text.replace("model_file = " + model_true, "model_file = " + model_true.replace("true", "init")))
+    out = _run("inversion_main", params)
+    conv = np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_2_iterations.txt"))
+    assert conv.shape == (3,) and conv[-1] < conv[0], out[-1500:]
