@@ -450,6 +450,102 @@ int slot_columns(fwi_ctx* c, int s, float* tm, float* compact, bool to_slot) {
     return FWI_OK;
 }
 
+// Seeded generator of the host-side random draws (splitmix64 stream).
+struct HostRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
+};
+
+// Blobs of the reference-style random boundary (modeling.cu:558-592):
+// Poisson-disk points with minimum distance `ratio` in the padded box
+// [0, (n-1) dh)^3 (Bridson's algorithm: 30 candidates in the shell
+// [ratio, 2 ratio) around a random active point, grid cells of ratio/sqrt(3)
+// so that the 5x5x5 neighbourhood covers the exclusion ball), keeping those
+// outside the inner box 0.5*nb*dh away from the faces; each gets a radius
+// r ~ U[0.2 ratio, ratio), an amplitude A ~ U[0.5 var_vp, var_vp) and a sign
+// (-1 with probability 1/3).  Output rows: xc, yc, zc, r, +-A.
+std::vector<float> make_blobs(uint64_t seed, int shot, int nxx, int nyy, int nzz, int nb, float dh, float ratio,
+                              float var_vp) {
+    HostRng rng{(seed * 0xD6E8FEB86659FD93ull) ^ ((uint64_t)(uint32_t)shot * 0xA0761D6478BD642Full) ^
+                0x2545F4914F6CDD1Dull};
+    const double xm = (double)((float)(nxx - 1) * dh), ym = (double)((float)(nyy - 1) * dh),
+                 zm = (double)((float)(nzz - 1) * dh);
+    const double rad = ratio, cell = rad / std::sqrt(3.0);
+    const int gx = std::max(1, (int)std::ceil(xm / cell)), gy = std::max(1, (int)std::ceil(ym / cell)),
+              gz = std::max(1, (int)std::ceil(zm / cell));
+    std::vector<int> grid((size_t)gx * gy * gz, -1);
+    std::vector<double> pts;  // x, y, z triples
+    std::vector<int> active;
+    auto gidx = [&](double x, double y, double z, int& i, int& j, int& k) {
+        i = std::min(gx - 1, (int)(x / cell));
+        j = std::min(gy - 1, (int)(y / cell));
+        k = std::min(gz - 1, (int)(z / cell));
+    };
+    auto valid = [&](double x, double y, double z) {
+        if (x < 0 || y < 0 || z < 0 || x >= xm || y >= ym || z >= zm) return false;
+        int i, j, k;
+        gidx(x, y, z, i, j, k);
+        for (int c = std::max(0, k - 2); c <= std::min(gz - 1, k + 2); ++c)
+            for (int b = std::max(0, j - 2); b <= std::min(gy - 1, j + 2); ++b)
+                for (int a = std::max(0, i - 2); a <= std::min(gx - 1, i + 2); ++a) {
+                    const int q = grid[(size_t)a + (size_t)gx * (b + (size_t)gy * c)];
+                    if (q < 0) continue;
+                    const double dx = pts[3 * q] - x, dy = pts[3 * q + 1] - y, dz = pts[3 * q + 2] - z;
+                    if (dx * dx + dy * dy + dz * dz < rad * rad) return false;
+                }
+        return true;
+    };
+    auto insert = [&](double x, double y, double z) {
+        int i, j, k;
+        gidx(x, y, z, i, j, k);
+        const int q = (int)(pts.size() / 3);
+        pts.push_back(x);
+        pts.push_back(y);
+        pts.push_back(z);
+        grid[(size_t)i + (size_t)gx * (j + (size_t)gy * k)] = q;
+        active.push_back(q);
+    };
+    if (xm > 0 && ym > 0 && zm > 0 && rad > 0) insert(rng.uniform() * xm, rng.uniform() * ym, rng.uniform() * zm);
+    while (!active.empty()) {
+        const size_t a = (size_t)(rng.uniform() * (double)active.size());
+        const int p = active[a];
+        bool found = false;
+        for (int t = 0; t < 30 && !found; ++t) {
+            const double u = 2.0 * rng.uniform() - 1.0, th = 2.0 * M_PI * rng.uniform();
+            const double rr = rad * (1.0 + rng.uniform()), sq = std::sqrt(std::max(0.0, 1.0 - u * u));
+            const double x = pts[3 * p] + rr * sq * std::cos(th), y = pts[3 * p + 1] + rr * sq * std::sin(th),
+                         z = pts[3 * p + 2] + rr * u;
+            if (valid(x, y, z)) {
+                insert(x, y, z);
+                found = true;
+            }
+        }
+        if (!found) {
+            active[a] = active.back();
+            active.pop_back();
+        }
+    }
+    const double fnb = (double)((float)nb * dh);
+    std::vector<float> out;
+    for (size_t q = 0; q < pts.size() / 3; ++q) {
+        const double x = pts[3 * q], y = pts[3 * q + 1], z = pts[3 * q + 2];
+        const bool inner = x > 0.5 * fnb && x < xm - 0.5 * fnb && y > 0.5 * fnb && y < ym - 0.5 * fnb &&
+                           z > 0.5 * fnb && z < zm - 0.5 * fnb;
+        if (inner) continue;
+        const float r = (float)(rad * (0.2 + 0.8 * rng.uniform()));
+        const float A = (float)((double)var_vp * (0.5 + 0.5 * rng.uniform()));
+        const float sign = rng.next() % 3 == 0 ? -1.0f : 1.0f;
+        out.insert(out.end(), {(float)x, (float)y, (float)z, r, sign * A});
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -576,6 +672,8 @@ int fwi_shot_random_boundary(fwi_ctx* c, int slot, uint64_t seed, int shot, floa
     if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
     if (!(amp >= 0.0f && amp < 1.0f)) return fail(c, FWI_ERR_ARG, "amp must be in [0, 1)");
     if (c->g.nb < 1) return fail(c, FWI_ERR_ARG, "random boundary needs nb >= 1");
+    if (ninterior(c->g) <= 0 || c->g.nxx <= 2 * c->g.nb || c->g.nyy <= 2 * c->g.nb)
+        return fail(c, FWI_ERR_ARG, "random boundary needs a non-empty interior");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, fwi::launch_random_boundary(c->vp + (long)slot * c->L.vol, c->L, c->g.nb, seed, shot, amp, vclamp,
                                           c->st));
@@ -586,6 +684,64 @@ int fwi_shot_random_boundary(fwi_ctx* c, int slot, uint64_t seed, int shot, floa
 
 int fwi_random_boundary(fwi_ctx* c, uint64_t seed, int shot, float amp, float vclamp) {
     return fwi_shot_random_boundary(c, 0, seed, shot, amp, vclamp);
+}
+
+int fwi_rbc_blobs(uint64_t seed, int shot, int nxx, int nyy, int nzz, int nb, float dh, float ratio, float var_vp,
+                  float* out, int cap, int* count) {
+    if (!count || nxx < 1 || nyy < 1 || nzz < 1 || nb < 0 || !(dh > 0) || !(ratio > 0) || !(var_vp >= 0))
+        return FWI_ERR_ARG;
+    const std::vector<float> b = make_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp);
+    *count = (int)(b.size() / 5);
+    if (out && cap >= *count) std::memcpy(out, b.data(), b.size() * sizeof(float));
+    return FWI_OK;
+}
+
+int fwi_shot_random_boundary_blobs(fwi_ctx* c, int slot, uint64_t seed, int shot, float ratio, float var_vp,
+                                   float vclamp) {
+    if (!c) return FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
+    if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
+    if (!(ratio > 0) || !(var_vp >= 0)) return fail(c, FWI_ERR_ARG, "ratio must be > 0 and var_vp >= 0");
+    if (c->g.nb < 1) return fail(c, FWI_ERR_ARG, "random boundary needs nb >= 1");
+    if (ninterior(c->g) <= 0 || c->g.nxx <= 2 * c->g.nb || c->g.nyy <= 2 * c->g.nb)
+        return fail(c, FWI_ERR_ARG, "random boundary needs a non-empty interior");
+    HIPCHK(c, hipSetDevice(c->device));
+    float* vol = c->vp + (long)slot * c->L.vol;
+    // the clamp bounds of random_boundary_gp: the extremes of the interior
+    // model (modeling.cu:164-171) -/+ var_vp; the interior is a sub-layout
+    // with the same pitches, so earlier boundaries do not widen the bounds
+    const int nb = c->g.nb;
+    fwi::Layout inner = c->L;
+    inner.nx -= 2 * nb;
+    inner.ny -= 2 * nb;
+    inner.nz -= 2 * nb;
+    HIPCHK(c, fwi::launch_stats(vol + c->L.at(nb, nb, nb), inner, c->red_partial, c->red_result, c->st));
+    double h[3];
+    HIPCHK(c, hipMemcpyAsync(h, c->red_result, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const std::vector<float> b = make_blobs(seed, shot, c->g.nxx, c->g.nyy, c->g.nzz, nb, c->g.dh, ratio, var_vp);
+    const int nblobs = (int)(b.size() / 5);
+    float* d_blobs = nullptr;
+    if (nblobs > 0) {
+        HIPCHK(c, hipMalloc((void**)&d_blobs, b.size() * sizeof(float)));
+        hipError_t e = hipMemcpy(d_blobs, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d_blobs);
+            return fail(c, FWI_ERR_HIP, std::string("blob upload: ") + hipGetErrorString(e));
+        }
+    }
+    hipError_t e = fwi::launch_blob_boundary(vol, c->L, nb, c->g.dh, seed, shot, var_vp, d_blobs, nblobs,
+                                             (float)h[1] - var_vp, (float)h[0] + var_vp, vclamp, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (d_blobs) (void)hipFree(d_blobs);
+    if (e != hipSuccess) return fail(c, FWI_ERR_HIP, std::string("blob boundary: ") + hipGetErrorString(e));
+    int rc = check_model_stats(c);
+    c->have_model = rc == FWI_OK;
+    return rc;
+}
+
+int fwi_random_boundary_blobs(fwi_ctx* c, uint64_t seed, int shot, float ratio, float var_vp, float vclamp) {
+    return fwi_shot_random_boundary_blobs(c, 0, seed, shot, ratio, var_vp, vclamp);
 }
 
 int fwi_set_wavelet(fwi_ctx* c, const float* w) {

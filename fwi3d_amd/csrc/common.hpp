@@ -109,6 +109,9 @@ hipError_t launch_to_host(const float* dev, float* host_layout, const Layout& L,
 hipError_t launch_expand(const float* interior_host_layout, float* dev, const Layout& L, int nb, hipStream_t st);
 hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t seed, int shot, float amp,
                                   float vclamp, hipStream_t st);
+hipError_t launch_blob_boundary(float* dev, const Layout& L, int nb, float dh, uint64_t seed, int shot,
+                                float var_vp, const float* blobs, int nblobs, float vlo, float vhi, float vclamp,
+                                hipStream_t st);
 hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, float value, hipStream_t st);
 hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st);
 hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st);

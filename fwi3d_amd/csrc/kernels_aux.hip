@@ -101,6 +101,158 @@ __global__ void __launch_bounds__(256) random_boundary_kernel(float* __restrict_
     dev[L.at(x, y, z)] = val;
 }
 
+// Reference-style random boundary (random_boundary_bg + random_boundary_gp,
+// modeling.cu:558-666), deterministic and in one launch.
+//  * background: each boundary cell starts from the nearest interior cell
+//    (so a repeated call starts afresh, as random_boundary_bg does); for
+//    each axis on which it lies in the boundary (x, then y, then z) it
+//    gets v += (f * var_vp) * n with f = 1 - d/nb and n ~ N(0, 1)
+//    approximated by the Irwin-Hall sum of 12 hashed uniforms minus 6;
+//  * blobs: in list order, v += amp * exp(-q/2), q = |X - c|^2 / r^2 (in
+//    double), for every blob with q <= 16 (4 radii; the reference sums every
+//    blob everywhere), then v is clamped to [vlo, vhi] after each blob;
+//  * finally v = min(v, vclamp).
+// Only the nb-cell boundary is written.  random_boundary_gp tests the cell
+// coordinate with X > nb*dh && X < x_max - nb*dh, which also admits the
+// outermost interior layer; that layer belongs to the imaged model and
+// would accumulate blobs over repeated calls, so it is left alone here.
+// A block (64 x 4 cells of one z-plane) first gathers, in list order, the
+// blobs whose 4-radius sphere meets its cells (wave ballots compact the
+// indices into LDS), then every cell applies them in that order.
+__device__ __forceinline__ double exp_det(double x) {
+    // e^x = 2^k e^r, k = floor(x / ln2 + 1/2), r = x - k ln2 in
+    // [-ln2/2, ln2/2]; e^r by its Taylor polynomial of degree 14 (Horner;
+    // truncation below 1e-17), exact scaling by 2^k.  Only basic double
+    // operations, so host and device results are identical.
+    constexpr double c[15] = {1.0,
+                              1.0,
+                              1.0 / 2.0,
+                              1.0 / 6.0,
+                              1.0 / 24.0,
+                              1.0 / 120.0,
+                              1.0 / 720.0,
+                              1.0 / 5040.0,
+                              1.0 / 40320.0,
+                              1.0 / 362880.0,
+                              1.0 / 3628800.0,
+                              1.0 / 39916800.0,
+                              1.0 / 479001600.0,
+                              1.0 / 6227020800.0,
+                              1.0 / 87178291200.0};
+    const double k = floor(x * 1.4426950408889634 + 0.5);
+    const double r = x - k * 0.6931471805599453;
+    double p = c[14];
+#pragma unroll
+    for (int n = 13; n >= 0; --n) p = c[n] + r * p;
+    return ldexp(p, (int)k);
+}
+
+__device__ __forceinline__ float irwin_hall_normal(uint64_t base, long cell, int axis) {
+    float s = 0.0f;
+    for (int m = 0; m < 12; ++m) {
+        const uint64_t h = splitmix64(base ^ ((uint64_t)cell * 36u + (uint64_t)(axis * 12 + m)));
+        s = s + (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+    }
+    return s - 6.0f;
+}
+
+__global__ void __launch_bounds__(256) blob_boundary_kernel(float* __restrict__ dev, Layout L, int nb, float dh,
+                                                            uint64_t base, float var_vp,
+                                                            const float* __restrict__ blobs, int nblobs, float vlo,
+                                                            float vhi, float vclamp) {
+    constexpr int CAP = 1024;
+    __shared__ int list[CAP];
+    __shared__ int wave_count[4];
+    __shared__ int count;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int wave = threadIdx.y, lane = threadIdx.x;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, z = blockIdx.z;
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    const bool zb = z < nb || z >= L.nz - nb;
+    // block-uniform: does any cell of the block lie in the boundary?
+    const bool any = zb || x0 < nb || x0 + 63 >= L.nx - nb || y0 < nb || y0 + 3 >= L.ny - nb;
+    if (!any) return;
+    const bool in_x = x >= nb && x < L.nx - nb, in_y = y >= nb && y < L.ny - nb, in_z = !zb;
+    const bool mine = x < L.nx && y < L.ny && !(in_x && in_y && in_z);
+
+    float v = 0.0f;
+    if (mine) {
+        v = dev[L.at(clampi(x, nb, L.nx - nb - 1), clampi(y, nb, L.ny - nb - 1), clampi(z, nb, L.nz - nb - 1))];
+        const long cell = (long)z + (long)x * L.nz + (long)y * L.nx * L.nz;
+        if (!in_x) {
+            const int d = x < nb ? x : L.nx - x - 1;
+            const float f = 1.0f - (float)d / (float)nb;
+            v = v + (f * var_vp) * irwin_hall_normal(base, cell, 0);
+        }
+        if (!in_y) {
+            const int d = y < nb ? y : L.ny - y - 1;
+            const float f = 1.0f - (float)d / (float)nb;
+            v = v + (f * var_vp) * irwin_hall_normal(base, cell, 1);
+        }
+        if (!in_z) {
+            const int d = z < nb ? z : L.nz - z - 1;
+            const float f = 1.0f - (float)d / (float)nb;
+            v = v + (f * var_vp) * irwin_hall_normal(base, cell, 2);
+        }
+    }
+    const float X = (float)x * dh, Y = (float)y * dh, Z = (float)z * dh;
+    // tile box in metres
+    const double bx0 = (double)((float)x0 * dh), bx1 = (double)((float)min(x0 + 63, L.nx - 1) * dh);
+    const double by0 = (double)((float)y0 * dh), by1 = (double)((float)min(y0 + 3, L.ny - 1) * dh);
+    const double bz = (double)Z;
+
+    auto apply = [&](int n) {
+        if (!mine) return;
+        for (int q = 0; q < n; ++q) {
+            const float* b = blobs + 5 * list[q];
+            const double r = (double)b[3];
+            const double ax = ((double)X - (double)b[0]) / r, ay = ((double)Y - (double)b[1]) / r,
+                         az = ((double)Z - (double)b[2]) / r;
+            const double qq = (ax * ax + ay * ay) + az * az;
+            if (qq <= 16.0) {
+                v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
+                v = v > vhi ? vhi : v;
+                v = v < vlo ? vlo : v;
+            }
+        }
+    };
+
+    if (tid == 0) count = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nblobs; c0 += 256) {
+        bool hit = false;
+        const int bi = c0 + tid;
+        if (bi < nblobs) {
+            const float* b = blobs + 5 * bi;
+            const double cx = b[0], cy = b[1], cz = b[2], reach = 4.0 * (double)b[3] * 1.0001;
+            const double dx = cx < bx0 ? bx0 - cx : (cx > bx1 ? cx - bx1 : 0.0);
+            const double dy = cy < by0 ? by0 - cy : (cy > by1 ? cy - by1 : 0.0);
+            const double dz = cz - bz;
+            hit = dx * dx + dy * dy + dz * dz <= reach * reach;
+        }
+        const uint64_t mask = __ballot(hit);
+        if (lane == 0) wave_count[wave] = __popcll(mask);
+        __syncthreads();
+        int off = count;
+        for (int w = 0; w < wave; ++w) off += wave_count[w];
+        if (hit) list[off + __popcll(mask & ((1ull << lane) - 1ull))] = bi;
+        __syncthreads();
+        if (tid == 0) count += wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+        __syncthreads();
+        if (count > CAP - 256) {  // flush before the next chunk could overflow
+            apply(count);
+            __syncthreads();
+            if (tid == 0) count = 0;
+            __syncthreads();
+        }
+    }
+    apply(count);
+    if (mine) {
+        if (vclamp > 0.0f && v > vclamp) v = vclamp;
+        dev[L.at(x, y, z)] = v;
+    }
+}
+
 // Source injection of the first step for every shot of a launch
 // (compute_pressure's atomicAdd, modeling.cu:347-348): shot s's field starts
 // at s*sstride and its source cell is off[s].
@@ -307,6 +459,16 @@ hipError_t launch_random_boundary(float* dev, const Layout& L, int nb, uint64_t 
                                   float vclamp, hipStream_t st) {
     dim3 grid((L.nx + 63) / 64, (L.ny + 3) / 4, L.nz);
     random_boundary_kernel<<<grid, dim3(64, 4), 0, st>>>(dev, L, nb, seed, shot, amp, vclamp);
+    return hipGetLastError();
+}
+
+hipError_t launch_blob_boundary(float* dev, const Layout& L, int nb, float dh, uint64_t seed, int shot,
+                                float var_vp, const float* blobs, int nblobs, float vlo, float vhi, float vclamp,
+                                hipStream_t st) {
+    const uint64_t base = (seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)shot * 0xD1B54A32D192ED03ull) ^
+                          0x5851F42D4C957F2Dull;
+    dim3 grid((L.nx + 63) / 64, (L.ny + 3) / 4, L.nz);
+    blob_boundary_kernel<<<grid, dim3(64, 4), 0, st>>>(dev, L, nb, dh, base, var_vp, blobs, nblobs, vlo, vhi, vclamp);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ EXPORTS = [
     "fwi_export_image", "fwi_export_gradient", "fwi_residual", "fwi_autotune", "fwi_get_tuning",
     "fwi_set_tuning", "fwi_set_graph", "fwi_last_pass_timing", "fwi_set_kernel_timing",
     "fwi_last_kernel_timing", "fwi_sync", "fwi_create_batch", "fwi_shot_random_boundary",
+    "fwi_random_boundary_blobs", "fwi_shot_random_boundary_blobs", "fwi_rbc_blobs",
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
     "fwi_copy_bandwidth",
@@ -44,6 +45,22 @@ class FwiError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (ERR.get(code, "FWI_ERR"), code, msg))
         self.code = code
+
+
+def rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp):
+    """The blob list of the reference-style random boundary, (n, 5) float32
+    rows (xc, yc, zc, r, amplitude); host-side, no device needed."""
+    L = load()
+    n = ctypes.c_int()
+    rc = L.fwi_rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp, None, 0, ctypes.byref(n))
+    if rc != OK:
+        raise FwiError(rc, "fwi_rbc_blobs")
+    out = np.zeros((n.value, 5), np.float32)
+    rc = L.fwi_rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp, out.ctypes.data_as(ctypes.c_void_p),
+                         n.value, ctypes.byref(n))
+    if rc != OK:
+        raise FwiError(rc, "fwi_rbc_blobs")
+    return out
 
 
 class Grid(ctypes.Structure):
@@ -101,6 +118,9 @@ def load():
         "fwi_sync": [P],
         "fwi_create_batch": [ctypes.POINTER(Grid), I, I, ctypes.POINTER(P)],
         "fwi_shot_random_boundary": [P, I, ctypes.c_uint64, I, F, F],
+        "fwi_random_boundary_blobs": [P, ctypes.c_uint64, I, F, F, F],
+        "fwi_shot_random_boundary_blobs": [P, I, ctypes.c_uint64, I, F, F, F],
+        "fwi_rbc_blobs": [ctypes.c_uint64, I, I, I, I, I, F, F, F, ctypes.c_void_p, I, ctypes.POINTER(I)],
         "fwi_shot_set_source": [P, I, I, I, I],
         "fwi_shot_set_receivers": [P, I, I, i32, i32, i32],
         "fwi_shot_get_seismogram": [P, I, f32],
@@ -216,6 +236,11 @@ class Context:
 
     def shot_random_boundary(self, slot, seed, shot, amp, vclamp):
         self._chk(self.L.fwi_shot_random_boundary(self.h, slot, seed, shot, amp, vclamp))
+
+    def random_boundary_blobs(self, seed, shot, ratio, var_vp, vclamp, slot=0):
+        """Reference-style random boundary: background noise plus Gaussian
+        blobs at Poisson-disk points (ratio in metres, var_vp in m/s)."""
+        self._chk(self.L.fwi_shot_random_boundary_blobs(self.h, slot, seed, shot, ratio, var_vp, vclamp))
 
     def set_shots_per_block(self, kind, sb):
         self._chk(self.L.fwi_set_shots_per_block(self.h, kind, sb))

@@ -109,6 +109,23 @@ int fwi_set_model_interior(fwi_ctx* ctx, const float* vp_interior);
  * and Poisson-disk noise are not reproducible. */
 int fwi_random_boundary(fwi_ctx* ctx, uint64_t seed, int shot, float amp, float vclamp);
 
+/* Reference-style random boundary (random_boundary_bg + random_boundary_gp,
+ * modeling.cu:558-666) in one fused launch instead of one launch per blob,
+ * deterministic for (seed, shot): boundary cells get per-axis noise
+ * (1 - d/nb) * var_vp * n, n ~ N(0,1) (Irwin-Hall of 12 hashed uniforms), then
+ * Gaussian blobs at Poisson-disk points (minimum distance `ratio` metres)
+ * outside the inner box, radius U[0.2, 1) ratio, amplitude +-U[0.5, 1) var_vp,
+ * each truncated at 4 radii, with the clamp to [vmin - var_vp, vmax + var_vp]
+ * after each blob, then min(v, vclamp).  Stands in for
+ * Modeling::set_random_boundary(Vp, ratio, varVp) (modeling.cu:558). */
+int fwi_random_boundary_blobs(fwi_ctx* ctx, uint64_t seed, int shot, float ratio, float var_vp, float vclamp);
+
+/* The blob list of fwi_random_boundary_blobs for a padded grid (host only,
+ * no device needed): *count rows of (xc, yc, zc, r, signed amplitude) in
+ * metres and m/s, written to out when cap >= *count. */
+int fwi_rbc_blobs(uint64_t seed, int shot, int nxx, int nyy, int nzz, int nb, float dh, float ratio, float var_vp,
+                  float* out, int cap, int* count);
+
 /* Source wavelet, nt samples (cudaMemcpy of d_wavelet, modeling.cu:54-56). */
 int fwi_set_wavelet(fwi_ctx* ctx, const float* wavelet);
 
@@ -188,6 +205,8 @@ int fwi_residual(fwi_ctx* ctx, const float* obs, double* sum_sq);
  * address one slot.  Entry points without a slot argument address slot 0. */
 int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out);
 int fwi_shot_random_boundary(fwi_ctx* ctx, int slot, uint64_t seed, int shot, float amp, float vclamp);
+int fwi_shot_random_boundary_blobs(fwi_ctx* ctx, int slot, uint64_t seed, int shot, float ratio, float var_vp,
+                                   float vclamp);
 int fwi_shot_set_source(fwi_ctx* ctx, int slot, int sIdx, int sIdy, int sIdz);
 int fwi_shot_set_receivers(fwi_ctx* ctx, int slot, int nrec, const int* rIdx, const int* rIdy, const int* rIdz);
 int fwi_shot_get_seismogram(fwi_ctx* ctx, int slot, float* out);

@@ -244,6 +244,93 @@ void oracle_random_boundary(float* vp, int nxx, int nyy, int nzz, int nb, uint64
             }
 }
 
+// Reference-style random boundary (random_boundary_bg + random_boundary_gp,
+// modeling.cu:558-666) as this build defines it deterministically:
+// per-axis background noise (1 - d/nb) * var_vp * n with n the Irwin-Hall
+// sum of 12 hashed uniforms minus 6 (x, then y, then z), then the blobs
+// (xc, yc, zc, r, amp rows) in order, each added as amp * e^(-q/2) with
+// q = |X - c|^2 / r^2 when q <= 16 and followed by the clamp to [vlo, vhi];
+// finally min(v, vclamp).  e^x is evaluated by exp_det below.
+static double exp_det(double x) {
+    // 2^k e^r with r = x - k ln2, e^r by its degree-14 Taylor polynomial
+    const double c[15] = {1.0,          1.0,           1.0 / 2.0,       1.0 / 6.0,        1.0 / 24.0,
+                          1.0 / 120.0,  1.0 / 720.0,   1.0 / 5040.0,    1.0 / 40320.0,    1.0 / 362880.0,
+                          1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0, 1.0 / 6227020800.0,
+                          1.0 / 87178291200.0};
+    const double k = std::floor(x * 1.4426950408889634 + 0.5);
+    const double r = x - k * 0.6931471805599453;
+    double p = c[14];
+    for (int n = 13; n >= 0; --n) p = c[n] + r * p;
+    return std::ldexp(p, (int)k);
+}
+
+double oracle_exp_det(double x) { return exp_det(x); }
+
+static float irwin_hall(uint64_t base, long cell, int axis) {
+    float s = 0.0f;
+    for (int m = 0; m < 12; ++m) {
+        const uint64_t h = splitmix64(base ^ ((uint64_t)cell * 36u + (uint64_t)(axis * 12 + m)));
+        s = s + (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+    }
+    return s - 6.0f;
+}
+
+float oracle_irwin_hall(uint64_t seed, int shot, long cell, int axis) {
+    const uint64_t base = (seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)shot * 0xD1B54A32D192ED03ull) ^
+                          0x5851F42D4C957F2Dull;
+    return irwin_hall(base, cell, axis);
+}
+
+void oracle_random_boundary_blobs(float* vp, int nxx, int nyy, int nzz, int nb, float dh, uint64_t seed, int shot,
+                                  float var_vp, const float* blobs, int nblobs, float vlo, float vhi, float vclamp) {
+    const uint64_t base = (seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)shot * 0xD1B54A32D192ED03ull) ^
+                          0x5851F42D4C957F2Dull;
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < nyy; k++)
+        for (int j = 0; j < nxx; j++)
+            for (int i = 0; i < nzz; i++) {
+                const bool in_x = j >= nb && j < nxx - nb, in_y = k >= nb && k < nyy - nb,
+                           in_z = i >= nb && i < nzz - nb;
+                if (in_x && in_y && in_z) continue;
+                const long c = lidx(i, j, k, nxx, nzz);
+                // start from the nearest interior cell (random_boundary_bg's index_ref)
+                const int ii = i < nb ? nb : (i > nzz - nb - 1 ? nzz - nb - 1 : i);
+                const int jj = j < nb ? nb : (j > nxx - nb - 1 ? nxx - nb - 1 : j);
+                const int kk = k < nb ? nb : (k > nyy - nb - 1 ? nyy - nb - 1 : k);
+                float v = vp[lidx(ii, jj, kk, nxx, nzz)];
+                if (!in_x) {
+                    const int d = j < nb ? j : nxx - j - 1;
+                    const float f = 1.0f - (float)d / (float)nb;
+                    v = v + (f * var_vp) * irwin_hall(base, c, 0);
+                }
+                if (!in_y) {
+                    const int d = k < nb ? k : nyy - k - 1;
+                    const float f = 1.0f - (float)d / (float)nb;
+                    v = v + (f * var_vp) * irwin_hall(base, c, 1);
+                }
+                if (!in_z) {
+                    const int d = i < nb ? i : nzz - i - 1;
+                    const float f = 1.0f - (float)d / (float)nb;
+                    v = v + (f * var_vp) * irwin_hall(base, c, 2);
+                }
+                const float X = (float)j * dh, Y = (float)k * dh, Z = (float)i * dh;
+                for (int q = 0; q < nblobs; ++q) {
+                    const float* b = blobs + 5 * q;
+                    const double r = (double)b[3];
+                    const double ax = ((double)X - (double)b[0]) / r, ay = ((double)Y - (double)b[1]) / r,
+                                 az = ((double)Z - (double)b[2]) / r;
+                    const double qq = (ax * ax + ay * ay) + az * az;
+                    if (qq <= 16.0) {
+                        v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
+                        v = v > vhi ? vhi : v;
+                        v = v < vlo ? vlo : v;
+                    }
+                }
+                if (vclamp > 0.0f && v > vclamp) v = vclamp;
+                vp[c] = v;
+            }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

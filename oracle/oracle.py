@@ -48,6 +48,12 @@ def lib():
     L.oracle_rbc_uniform.argtypes = [ctypes.c_uint64, _c_int, _c_long]
     L.oracle_rbc_uniform.restype = _c_float
     L.oracle_random_boundary.argtypes = [_f32p] + [_c_int] * 4 + [ctypes.c_uint64, _c_int, _c_float, _c_float]
+    L.oracle_random_boundary_blobs.argtypes = [_f32p] + [_c_int] * 4 + [_c_float, ctypes.c_uint64, _c_int,
+                                                                        _c_float, _f32p, _c_int] + [_c_float] * 3
+    L.oracle_exp_det.argtypes = [ctypes.c_double]
+    L.oracle_exp_det.restype = ctypes.c_double
+    L.oracle_irwin_hall.argtypes = [ctypes.c_uint64, _c_int, _c_long, _c_int]
+    L.oracle_irwin_hall.restype = _c_float
     fwd = [_f32p, _c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_int, _c_int, _f32p,
            _c_int, _c_int, _c_int, _c_int, _i32p, _i32p, _i32p, _c_int, vp, vp, vp, _c_int, _c_int]
     L.oracle_forward_f32.argtypes = fwd + [_f32p, _f32p, vp]
@@ -145,6 +151,25 @@ def random_boundary(vp_pad, nb, seed, shot, amp, vclamp):
     nyy, nxx, nzz = out.shape
     lib().oracle_random_boundary(out, nxx, nyy, nzz, nb, seed, shot, amp, vclamp)
     return out
+
+
+def random_boundary_blobs(vp_pad, nb, dh, seed, shot, var_vp, blobs, vlo, vhi, vclamp):
+    """Reference-style random boundary (background noise + blobs, see the C
+    source) over an edge-padded model; blobs is an (n, 5) float32 array of
+    (xc, yc, zc, r, amplitude).  Returns a copy."""
+    out = np.array(vp_pad, np.float32, copy=True, order="C")
+    nyy, nxx, nzz = out.shape
+    b = np.ascontiguousarray(np.asarray(blobs, np.float32).reshape(-1, 5))
+    lib().oracle_random_boundary_blobs(out, nxx, nyy, nzz, nb, dh, seed, shot, var_vp, b, len(b), vlo, vhi, vclamp)
+    return out
+
+
+def exp_det(x):
+    return lib().oracle_exp_det(float(x))
+
+
+def irwin_hall(seed, shot, cell, axis):
+    return lib().oracle_irwin_hall(seed, shot, cell, axis)
 
 
 def _rec_arrays(rec):

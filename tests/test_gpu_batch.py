@@ -5,8 +5,7 @@ one step kernel per time step.  Forward seismograms are per-shot independent
 and must match single-shot runs bit for bit.  The stacked RTM image sums the
 shots per time step instead of shot after shot: it equals the oracle run in
 that order bit for bit, and differs from sequential per-shot accumulation by
-rounding only (held to relative L2 <= 1e-4, the bound the batched RTM
-target states)."""
+rounding only (held to relative L2 <= 1e-4 for the batched image)."""
 import numpy as np
 import pytest
 
@@ -219,6 +218,26 @@ def test_batched_models_match_oracle(fl):
         want = O.random_boundary(pad, case.nb, 5, 10 + s, 0.6, case.vclamp)
         assert np.array_equal(c.get_volume(fl.VOL_MODEL, slot=s), want)
     c.close()
+
+
+def test_batched_blob_boundaries_match_oracle(fl):
+    """Per-slot reference-style boundaries: each slot's own shot index."""
+    case = Case(amp=0.0)
+    S, ratio, var_vp = 2, 40.0, 800.0
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    for s in range(S):
+        c.random_boundary_blobs(3, 20 + s, ratio, var_vp, float(case.vclamp), slot=s)
+    pad = O.expand_boundary(case.interior, case.nb)
+    vlo = np.float32(case.interior.min()) - np.float32(var_vp)
+    vhi = np.float32(case.interior.max()) + np.float32(var_vp)
+    got = [c.get_volume(fl.VOL_MODEL, slot=s) for s in range(S)]
+    c.close()
+    for s in range(S):
+        blobs = fl.rbc_blobs(3, 20 + s, case.nxx, case.nyy, case.nzz, case.nb, case.dh, ratio, var_vp)
+        want = O.random_boundary_blobs(pad, case.nb, case.dh, 3, 20 + s, var_vp, blobs, vlo, vhi, case.vclamp)
+        assert np.array_equal(got[s], want), s
+    assert not np.array_equal(got[0], got[1])
 
 
 def test_batched_residual_and_fwi_gradient(fl):

@@ -12,7 +12,7 @@ from tests.cases import Case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TOL_FP32 = 1e-5  # north-star FP32 parity bound (relative L2)
+TOL_FP32 = 1e-5  # FP32 parity bound against the oracle (relative L2)
 
 
 @pytest.fixture(scope="module")
@@ -191,6 +191,33 @@ def test_model_setup_matches_oracle(fl):
         c.random_boundary(11, 5, 0.6, float(case.vclamp))
         want = O.random_boundary(O.expand_boundary(case.interior, case.nb), case.nb, 11, 5, 0.6, case.vclamp)
         assert np.array_equal(c.get_volume(fl.VOL_MODEL), want)
+
+
+@pytest.mark.parametrize("ratio,var_vp,dims", [(30.0, 1000.0, (40, 36, 32)), (8.0, 1500.0, (24, 20, 16))])
+def test_blob_random_boundary_bit_exact(fl, ratio, var_vp, dims):
+    """Reference-style boundary (noise + Poisson-disk blobs) against the
+    oracle with the library's own blob list; the small ratio puts more than
+    a thousand blobs near some blocks (the gathered list is flushed).  A
+    second call starts afresh from the interior."""
+    nx, ny, nz = dims
+    case = Case(nx=nx, ny=ny, nz=nz, amp=0.0)
+    with case.context(fl.Context) as c:
+        c.set_model_interior(case.interior)
+        c.random_boundary_blobs(7, 2, ratio, var_vp, float(case.vclamp))
+        got = c.get_volume(fl.VOL_MODEL)
+        c.random_boundary_blobs(7, 2, ratio, var_vp, float(case.vclamp))
+        again = c.get_volume(fl.VOL_MODEL)
+    blobs = fl.rbc_blobs(7, 2, case.nxx, case.nyy, case.nzz, case.nb, case.dh, ratio, var_vp)
+    vlo = np.float32(case.interior.min()) - np.float32(var_vp)
+    vhi = np.float32(case.interior.max()) + np.float32(var_vp)
+    pad = O.expand_boundary(case.interior, case.nb)
+    want = O.random_boundary_blobs(pad, case.nb, case.dh, 7, 2, var_vp, blobs, vlo, vhi, case.vclamp)
+    assert len(blobs) > 100
+    assert np.array_equal(got, want)
+    assert np.array_equal(again, got)
+    nb = case.nb
+    assert np.array_equal(got[nb:-nb, nb:-nb, nb:-nb], case.interior)
+    assert np.abs(got - pad).max() > 0.5 * var_vp
 
 
 def test_exports_match_oracle(fl):

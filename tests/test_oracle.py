@@ -32,6 +32,10 @@ function is checked against what the mathematics fixes:
   model_step                 1/sqrt(m - a g) in FP64, rounded
   random boundary            interior untouched; |v/v_face - 1| <= amp*ramp;
                              clamp; seeded determinism; U[-1,1) moments
+  blob random boundary       exp_det = math.exp to 1e-15; Irwin-Hall normal
+                             moments and bounds; one blob on a constant model
+                             = v0 + A e^(-q/2) cut at 4 radii (closed form);
+                             interior untouched
 """
 import math
 
@@ -376,3 +380,50 @@ def test_random_boundary_invariants():
     u = np.array([O.rbc_uniform(9, 0, q) for q in range(20000)])
     assert u.min() >= -1.0 and u.max() < 1.0
     assert abs(u.mean()) < 0.02 and abs(u.var() - 1 / 3) < 0.02
+
+
+def test_exp_det_matches_math_exp():
+    for x in np.linspace(-9.0, 0.0, 2001):
+        e = math.exp(x)
+        assert abs(O.exp_det(x) - e) <= 2e-15 * e, x
+
+
+def test_irwin_hall_normal_moments():
+    v = np.array([O.irwin_hall(5, 1, c, a) for c in range(6000) for a in range(3)], np.float64)
+    assert abs(v.mean()) < 0.03 and abs(v.std() - 1.0) < 0.03
+    assert v.min() >= -6.0 and v.max() <= 6.0
+    assert abs((v ** 4).mean() - 2.9) < 0.2  # kurtosis of the 12-term sum: 3 - 6/60
+
+
+def test_blob_boundary_single_blob_closed_form():
+    """No background noise (var_vp = 0), one blob, wide clamp bounds: boundary
+    cells get v0 + A exp(-q/2) within 4 radii, everything else is unchanged."""
+    n, nb, dh = 24, 6, 10.0
+    vp = np.full((n, n, n), 2000.0, np.float32)
+    blob = np.array([[30.0, 45.0, 200.0, 35.0, -300.0]], np.float32)  # near the x = 0 face
+    out = O.random_boundary_blobs(vp, nb, dh, 3, 0, 0.0, blob, 0.0, 1e9, 0.0)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    X, Y, Z = (j * np.float32(dh)).astype(np.float64), (k * np.float32(dh)).astype(np.float64), \
+        (i * np.float32(dh)).astype(np.float64)
+    q = ((X - 30.0) / 35.0) ** 2 + ((Y - 45.0) / 35.0) ** 2 + ((Z - 200.0) / 35.0) ** 2
+    inner = (j >= nb) & (j < n - nb) & (k >= nb) & (k < n - nb) & (i >= nb) & (i < n - nb)
+    want = np.where(~inner & (q <= 16.0), np.float32(2000.0) + (-300.0 * np.exp(-0.5 * q)).astype(np.float32),
+                    np.float32(2000.0)).astype(np.float32)
+    diff = np.abs(out.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert diff.max() <= 1
+    assert np.array_equal(out[inner], vp[inner])
+    assert (out < 2000.0).sum() > 50
+
+
+def test_blob_boundary_restarts_from_the_interior():
+    """A second application gives the same volume (the base value is read
+    from the nearest interior cell, not from the boundary cell itself)."""
+    rng = np.random.default_rng(2)
+    n, nb = 20, 5
+    vp = O.expand_boundary(rng.uniform(1500, 2500, (n - 2 * nb,) * 3).astype(np.float32), nb)
+    blobs = np.array([[0.0, 0.0, 0.0, 30.0, 400.0], [190.0, 100.0, 50.0, 20.0, -300.0]], np.float32)
+    a = O.random_boundary_blobs(vp, nb, 10.0, 9, 1, 200.0, blobs, 1000.0, 3000.0, 0.0)
+    b = O.random_boundary_blobs(a, nb, 10.0, 9, 1, 200.0, blobs, 1000.0, 3000.0, 0.0)
+    assert np.array_equal(a, b)
+    assert np.array_equal(a[nb:-nb, nb:-nb, nb:-nb], vp[nb:-nb, nb:-nb, nb:-nb])
+    assert not np.array_equal(a, vp)
