@@ -124,12 +124,13 @@ class Workload:
 
     # -- reference file layout ---------------------------------------------
     def write(self, folder, prefix=None, model=None, kind="true", max_iteration=5, batch_shots=1, depth_mask=40.0,
-              abc_factor=0.004):
+              abc_factor=0.004, rbc_style=None):
         """Write model, SPS/RPS and parameters.txt under `folder`; returns the
         parameters.txt path.  `model` replaces the interior written as the
         model file (e.g. a smoothed starting model for inversion, named with
         `kind`).  The reference reads one RPS file for all shots, so shot 0's
-        receivers are written."""
+        receivers are written.  `rbc_style` ("blobs" or "ramp") is written
+        when given (the drivers default to blobs)."""
         prefix = prefix or self.name
         os.makedirs(folder, exist_ok=True)
         for sub in ("inputs/models", "inputs/geometry", "inputs/data", "outputs/models", "outputs/residuo",
@@ -150,6 +151,8 @@ class Workload:
             nx=self.nx, ny=self.ny, nz=self.nz, dh=self.dh, model=model, sps=sps, rps=rps, nt=self.nt,
             dt=self.dt, fmax=self.fmax, length=length, folder=folder, nrec=nrec, dtus=dtus, seed=self.rbc_seed,
             amp=self.rbc_amp, maxit=max_iteration, batch=batch_shots, zmask=depth_mask, factor=abc_factor)
+        if rbc_style is not None:
+            text += "rbc_style = %s\n" % rbc_style
         path = os.path.join(folder, "parameters.txt")
         with open(path, "w") as f:
             f.write(text)

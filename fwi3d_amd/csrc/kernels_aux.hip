@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(256) random_boundary_kernel(float* __restrict_
 //    approximated by the Irwin-Hall sum of 12 hashed uniforms minus 6;
 //  * blobs: in list order, v += amp * exp(-q/2), q = |X - c|^2 / r^2 (in
 //    double), for every blob with q <= 16 (4 radii; the reference sums every
-//    blob everywhere), then v is clamped to [vlo, vhi] after each blob;
+//    blob everywhere, and a blob beyond 4 radii adds under amp * e^-8), and
+//    v is clamped to [vlo, vhi] after each blob, in reach or not;
 //  * finally v = min(v, vclamp).
 // Only the nb-cell boundary is written.  random_boundary_gp tests the cell
 // coordinate with X > nb*dh && X < x_max - nb*dh, which also admits the
@@ -201,16 +202,28 @@ __global__ void __launch_bounds__(256) blob_boundary_kernel(float* __restrict__ 
     const double by0 = (double)((float)y0 * dh), by1 = (double)((float)min(y0 + 3, L.ny - 1) * dh);
     const double bz = (double)Z;
 
+    // The clamp follows every blob, in reach or not.  Once blob 0 has passed
+    // v stays inside [vlo, vhi], so a clamp where no blob is in reach is a
+    // no-op except at blob 0: the first gathered list applies it up front
+    // when blob 0 is not in it (also for a block that gathered nothing).
+    bool started = false;
     auto apply = [&](int n) {
         if (!mine) return;
+        if (!started) {
+            started = true;
+            if (nblobs > 0 && (n == 0 || list[0] != 0)) {
+                v = v > vhi ? vhi : v;
+                v = v < vlo ? vlo : v;
+            }
+        }
         for (int q = 0; q < n; ++q) {
             const float* b = blobs + 5 * list[q];
             const double r = (double)b[3];
             const double ax = ((double)X - (double)b[0]) / r, ay = ((double)Y - (double)b[1]) / r,
                          az = ((double)Z - (double)b[2]) / r;
             const double qq = (ax * ax + ay * ay) + az * az;
-            if (qq <= 16.0) {
-                v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
+            if (qq <= 16.0) v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
+            if (qq <= 16.0 || list[q] == 0) {
                 v = v > vhi ? vhi : v;
                 v = v < vlo ? vlo : v;
             }

@@ -108,8 +108,7 @@ void Inversion::compute_gradient() {
         place_shots(rbc, rbc_nb, srcId, count);
         for (int slot = 0; slot < batch; slot++) {  // update_RBC, :204-216
             const int shot = srcId + (slot < count ? slot : 0);
-            check(fwi_shot_random_boundary(rbc, slot, rbc_seed, shot, rbc_amplitude, cfl_velocity()),
-                  "fwi_shot_random_boundary");
+            random_boundary(rbc, slot, shot, rbc_ratio, rbc_varVp);
             if (slot < count) {
                 check(fwi_shot_get_backward_data(ctx, slot, residual.data()), "fwi_shot_get_backward_data");
             } else {

@@ -41,8 +41,7 @@ void Migration::forward_propagation() {
     initialization();
     for (int slot = 0; slot < batch; slot++) {
         const int shot = first_shot + (slot < shot_count ? slot : 0);
-        check(fwi_shot_random_boundary(ctx, slot, rbc_seed, shot, rbc_amplitude, cfl_velocity()),
-              "fwi_shot_random_boundary");
+        random_boundary(ctx, slot, shot, rbc_ratio, rbc_varVp);
     }
     forward_solver();
 }

@@ -54,6 +54,10 @@ void Modeling::set_build_parameters() {
     batch = std::max(1, std::stoi(catch_parameter_or("batch_shots", parameters, "1")));
     rbc_seed = std::stoull(catch_parameter_or("rbc_seed", parameters, "20261203"));
     rbc_amplitude = std::stof(catch_parameter_or("rbc_amplitude", parameters, "0.6"));
+    const std::string style = catch_parameter_or("rbc_style", parameters, "blobs");
+    if (style == "blobs") rbc_blobs = true;
+    else if (style == "ramp") rbc_blobs = false;
+    else throw std::invalid_argument("rbc_style must be 'blobs' or 'ramp', not '" + style + "'");
     const std::string a = catch_parameter_or("arithmetic", parameters, "reference");
     if (a == "reference") arith = FWI_ARITH_REFERENCE;
     else if (a == "fp32") arith = FWI_ARITH_FP32;
@@ -140,6 +144,16 @@ void Modeling::create_context(int boundary_cells, fwi_ctx** out) {
 }
 
 float Modeling::cfl_velocity() const { return 0.45f * dh / dt; }
+
+// modeling.cu:558-592 (blobs) or the ramp boundary, seeded by (rbc_seed, shot)
+void Modeling::random_boundary(fwi_ctx* c, int slot, int shot, float ratio, float varVp) {
+    if (rbc_blobs)
+        check(fwi_shot_random_boundary_blobs(c, slot, rbc_seed, shot, ratio, varVp, cfl_velocity()),
+              "fwi_shot_random_boundary_blobs");
+    else
+        check(fwi_shot_random_boundary(c, slot, rbc_seed, shot, rbc_amplitude, cfl_velocity()),
+              "fwi_shot_random_boundary");
+}
 
 // modeling.cu:270-272, 286-288
 void Modeling::grid_index(float x, float y, float z, int* ix, int* iy, int* iz) const {

@@ -11,9 +11,13 @@
 //   batch_shots   = N   shots advanced together by one launch per time step
 //                       (default 1: one shot after the other, as the
 //                       reference's shot loop)
+//   rbc_style     = blobs | ramp   random boundary of migration and
+//                       inversion (default blobs: set_random_boundary's
+//                       background noise and Gaussian blobs, sized by the
+//                       {mig,inv}_rbc_ratio and _varVp keys; ramp: uniform
+//                       noise of +-A*v growing linearly to the outer edge)
 //   rbc_seed      = S   seed of the random boundary (default 20261203)
-//   rbc_amplitude = A   random boundary amplitude, +-A*v at the outer edge
-//                       (default 0.6)
+//   rbc_amplitude = A   ramp amplitude (default 0.6)
 //   arithmetic    = reference | fp32   (default reference)
 //   backward_corrections = comma-separated subset of
 //                   gradient_dyy, subtract_source, aligned_imaging
@@ -56,6 +60,7 @@ protected:
     int first_shot = 0, shot_count = 1;
     uint64_t rbc_seed = 20261203ull;
     float rbc_amplitude = 0.6f;
+    bool rbc_blobs = true;  // rbc_style
     int arith = FWI_ARITH_REFERENCE;
     int physics = 0;  // FWI_PHYS_* bits for the backward passes
 
@@ -71,6 +76,8 @@ protected:
     void check(int rc, const std::string& what) const;
     void grid_index(float x, float y, float z, int* ix, int* iy, int* iz) const;
     float cfl_velocity() const;
+    // the random boundary of rbc_style on slot `slot` of c for shot `shot`
+    void random_boundary(fwi_ctx* c, int slot, int shot, float ratio, float varVp);
 
 public:
     int srcId = 0;

@@ -245,12 +245,13 @@ void oracle_random_boundary(float* vp, int nxx, int nyy, int nzz, int nb, uint64
 }
 
 // Reference-style random boundary (random_boundary_bg + random_boundary_gp,
-// modeling.cu:558-666) as this build defines it deterministically:
-// per-axis background noise (1 - d/nb) * var_vp * n with n the Irwin-Hall
-// sum of 12 hashed uniforms minus 6 (x, then y, then z), then the blobs
-// (xc, yc, zc, r, amp rows) in order, each added as amp * e^(-q/2) with
-// q = |X - c|^2 / r^2 when q <= 16 and followed by the clamp to [vlo, vhi];
-// finally min(v, vclamp).  e^x is evaluated by exp_det below.
+// modeling.cu:558-666) made deterministic: each boundary cell starts from
+// the nearest interior cell and gets per-axis background noise
+// (1 - d/nb) * var_vp * n with n the Irwin-Hall sum of 12 hashed uniforms
+// minus 6 (x, then y, then z); then the blobs (xc, yc, zc, r, amp rows) in
+// order, each added as amp * e^(-q/2) with q = |X - c|^2 / r^2 when q <= 16,
+// every blob followed by the clamp to [vlo, vhi]; finally min(v, vclamp).
+// e^x is evaluated by exp_det below.
 static double exp_det(double x) {
     // 2^k e^r with r = x - k ln2, e^r by its degree-14 Taylor polynomial
     const double c[15] = {1.0,          1.0,           1.0 / 2.0,       1.0 / 6.0,        1.0 / 24.0,
@@ -320,11 +321,10 @@ void oracle_random_boundary_blobs(float* vp, int nxx, int nyy, int nzz, int nb, 
                     const double ax = ((double)X - (double)b[0]) / r, ay = ((double)Y - (double)b[1]) / r,
                                  az = ((double)Z - (double)b[2]) / r;
                     const double qq = (ax * ax + ay * ay) + az * az;
-                    if (qq <= 16.0) {
-                        v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
-                        v = v > vhi ? vhi : v;
-                        v = v < vlo ? vlo : v;
-                    }
+                    if (qq <= 16.0) v = v + (float)((double)b[4] * exp_det(-0.5 * qq));
+                    // random_boundary_gp clamps after every blob, near or far
+                    v = v > vhi ? vhi : v;
+                    v = v < vlo ? vlo : v;
                 }
                 if (vclamp > 0.0f && v > vclamp) v = vclamp;
                 vp[c] = v;

@@ -115,6 +115,31 @@ def test_migration_main_backward_corrections(tiny, tmp_path):
     assert r.returncode != 0 and "bogus" in r.stdout
 
 
+def test_migration_main_rbc_style(tiny, tmp_path):
+    """rbc_style selects the boundary: the default (the reference's blobs)
+    and the ramp give different finite sections; an unknown style is refused."""
+    w = tiny
+    imgs = []
+    for name, style in (("default", None), ("ramp", "ramp"), ("blobs", "blobs")):
+        folder = str(tmp_path / name)
+        params = w.write(folder, batch_shots=3, rbc_style=style)
+        _run("modeling_main", params)
+        _run("migration_main", params)
+        f = os.path.join(folder, "outputs/seismic", "RTM_section_45Hz_%dx%dx%d_10m.bin" % (w.nz, w.nx, w.ny))
+        imgs.append(np.fromfile(f, np.float32))
+        assert np.isfinite(imgs[-1]).all() and np.abs(imgs[-1]).max() > 0
+    assert np.array_equal(imgs[0], imgs[2])
+    assert rel_l2(imgs[1], imgs[0]) > 1e-3
+    with open(params, "a") as f:
+        f.write("rbc_style = bogus\n")
+    text = open(params).read().replace("rbc_style = blobs\n", "")
+    with open(params, "w") as f:
+        f.write(text)
+    r = subprocess.run([os.path.join(BIN, "migration_main"), params], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=240)
+    assert r.returncode != 0 and "bogus" in r.stdout
+
+
 def test_full_workflow_in_cpp(tiny, tmp_path):
     """The reference workflow end to end with no Python step: setup_main
     (acquisition + smoothed initial model), modeling_main, preprocess_main

@@ -413,6 +413,12 @@ def test_blob_boundary_single_blob_closed_form():
     assert diff.max() <= 1
     assert np.array_equal(out[inner], vp[inner])
     assert (out < 2000.0).sum() > 50
+    # the clamp follows every blob, also where none is in reach
+    far = np.array([[30.0, 45.0, 200.0, 1.0, 50.0]], np.float32)
+    out = O.random_boundary_blobs(vp, nb, dh, 3, 0, 0.0, far, 0.0, 1900.0, 0.0)
+    assert np.all(out[~inner] == np.float32(1900.0)) and np.array_equal(out[inner], vp[inner])
+    out = O.random_boundary_blobs(vp, nb, dh, 3, 0, 0.0, far[:0], 0.0, 1900.0, 0.0)
+    assert np.array_equal(out, vp)  # no blob, no clamp
 
 
 def test_blob_boundary_restarts_from_the_interior():
