@@ -109,6 +109,63 @@ def cpu_vs_gpu_two_layer(lib, configs):
             "seismogram_bit_exact": bool(np.array_equal(g, seis))}
 
 
+def blob_boundary(w, ctx, lib, shots=3, ratio=100.0, var_vp=1000.0):
+    """The reference-style random boundary (modeling.cu:558-673) on the
+    workload grid with the reference parameters.txt's mig_rbc_ratio = 100 m
+    and mig_rbc_varVp = 1000 m/s: the host Poisson-disk list alone, and the
+    whole library call (list, upload, one fused launch), per shot.  The
+    reference launches one full-volume kernel per blob; its HBM floor (read
+    and write every padded cell once per blob at 8 TB/s) is reported beside."""
+    t_list, nblobs = [], []
+    for s in range(shots):
+        t0 = time.perf_counter()
+        b = lib.rbc_blobs(w.rbc_seed, s, w.nxx, w.nyy, w.nzz, w.nb, w.dh, ratio, var_vp)
+        t_list.append(time.perf_counter() - t0)
+        nblobs.append(len(b))
+    t_call = []
+    for s in range(shots):
+        ctx.set_model_interior(w.interior)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.random_boundary_blobs(w.rbc_seed, s, ratio, var_vp, w.vclamp)
+        ctx.sync()
+        t_call.append(time.perf_counter() - t0)
+    floor = float(np.mean(nblobs)) * 8.0 * w.nxx * w.nyy * w.nzz / (HBM_PEAK_GBS * 1e9)
+    log("blob boundary: %d blobs, list %.0f ms, call %.0f ms" % (nblobs[0], 1e3 * np.mean(t_list),
+                                                                  1e3 * np.mean(t_call)))
+    return {"ratio_m": ratio, "var_vp": var_vp, "blobs": nblobs, "host_list_ms": round(1e3 * float(np.mean(t_list)), 1),
+            "call_ms": round(1e3 * float(np.mean(t_call)), 1),
+            "kernel_ms_approx": round(1e3 * float(np.mean(t_call) - np.mean(t_list)), 1),
+            "reference_launch_pattern_hbm_floor_ms": round(1e3 * floor, 1)}
+
+
+def dataset_tools(w):
+    """The C++ initial-model filter (setup_main) against scipy's
+    gaussian_filter, which the reference's run/generate_setup.py:64-66 calls,
+    on the workload's interior slowness (z fastest) with sigma = 3 cells (the
+    script's) and 15 cells (150 m); both must agree bit for bit."""
+    import ctypes
+    from scipy.ndimage import gaussian_filter
+    L = ctypes.CDLL(os.path.join(ROOT, "fwi3d_amd", "libfwi3d_host.so"))
+    f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.fwi3d_host_gaussian_filter3d.argtypes = [f32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    ny, nx, nz = w.interior.shape
+    slow = np.ascontiguousarray(1.0 / w.interior).reshape(-1)  # x-y-z array in memory = z fastest
+    out = {"grid_zxy": [nz, nx, ny], "threads": os.environ.get("OMP_NUM_THREADS", str(os.cpu_count()))}
+    for sigma in (3.0, 15.0):
+        t0 = time.perf_counter()
+        want = gaussian_filter(slow.reshape((nz, nx, ny), order="F"), sigma).flatten("F")
+        t_py = time.perf_counter() - t0
+        got = slow.copy()
+        t0 = time.perf_counter()
+        L.fwi3d_host_gaussian_filter3d(got, nz, nx, ny, sigma)
+        t_cpp = time.perf_counter() - t0
+        out["sigma_%g" % sigma] = {"cpp_s": round(t_cpp, 3), "scipy_s": round(t_py, 3),
+                                   "bit_exact": bool(np.array_equal(got, want))}
+        log("gaussian filter sigma %g: C++ %.2f s, scipy %.2f s" % (sigma, t_cpp, t_py))
+    return out
+
+
 def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=True):
     """RTM of the first `nshots` shots of the 4x4 surface survey (each shot its
     own random boundary and its own 201x201 receiver patch clipped at the
@@ -222,6 +279,8 @@ def main():
                          "(BY, RPT, z-chunks, shots per block)")
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="capture the batched passes' time loops in hipGraphs (1) or launch on the stream (0)")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the blob-boundary and dataset-tool timings")
     ap.add_argument("--batch-shots", type=int, default=16,
                     help="also time this many shots of the 4x4 surface survey as one batch against the "
                          "per-shot path (0: skip; 16: the full survey)")
@@ -338,6 +397,10 @@ def main():
     # the timed shot's padded model (the per-shot leg below replaces its boundary)
     ctx_vp = ctx.get_volume(lib.VOL_MODEL) if not args.no_cpu_baseline else None
 
+    aux = None
+    if not args.no_aux and not args.pass_steps:
+        aux = {"blob_random_boundary": blob_boundary(w, ctx, lib), "dataset_tools": dataset_tools(w)}
+
     batched = None
     if args.batch_shots > 0:
         batched = batched_rtm(w, ctx, args.batch_shots, lib, configs, args.batch_tuning, args.pass_steps,
@@ -400,6 +463,7 @@ def main():
         "forward_pass_ms": round(fwd_pass / args.steps, 2),
         "rtm_backward_pass_ms": round(bwd_pass / args.steps, 2),
         "batched_rtm": batched,
+        "widened_rows": aux,
         "fwi_time_to_misfit": fwi_res,
         "copy_kernel_GBs": round(copy_gbs, 1),
     }

@@ -69,16 +69,24 @@ inline long reflect(long q, long n) {
 }
 
 // One 1-D pass along the axis with stride `s` and length n for every line;
-// `lines` lists the first element of each line.
+// `lines` lists the first element of each line.  Lines are independent, so
+// they are shared among OpenMP threads (each with its own line buffer); the
+// result does not depend on the thread count.
 void pass1d(float* v, long n, long s, const std::vector<long>& lines, const std::vector<double>& w, int r) {
-    std::vector<double> buf(n + 2 * r);
-    for (long base : lines) {
-        for (long q = -r; q < n + r; q++) buf[q + r] = (double)v[base + reflect(q, n) * s];
-        for (long i = 0; i < n; i++) {
-            const double* x = buf.data() + i + r;
-            double t = x[0] * w[r];
-            for (int j = r; j >= 1; j--) t += (x[-j] + x[j]) * w[r - j];
-            v[base + i * s] = (float)t;
+    const long nlines = (long)lines.size();
+#pragma omp parallel
+    {
+        std::vector<double> buf(n + 2 * r);
+#pragma omp for schedule(static)
+        for (long l = 0; l < nlines; l++) {
+            const long base = lines[l];
+            for (long q = -r; q < n + r; q++) buf[q + r] = (double)v[base + reflect(q, n) * s];
+            for (long i = 0; i < n; i++) {
+                const double* x = buf.data() + i + r;
+                double t = x[0] * w[r];
+                for (int j = r; j >= 1; j--) t += (x[-j] + x[j]) * w[r - j];
+                v[base + i * s] = (float)t;
+            }
         }
     }
 }

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <fstream>
 #include <iostream>
+#include <stdexcept>
 
 #include "admin.hpp"
 
@@ -35,6 +36,10 @@ void Inversion::set_parameters() {
     residuo_folder = catch_parameter("inv_residuo_folder", parameters);
     max_iteration = std::stoi(catch_parameter("max_iteration", parameters));
     iteration = 0;
+    const std::string upload = catch_parameter_or("fwi_model_upload", parameters, "updated");
+    if (upload == "updated") upload_updated = true;
+    else if (upload == "last_trial") upload_updated = false;
+    else throw std::invalid_argument("fwi_model_upload must be 'updated' or 'last_trial', not '" + upload + "'");
 
     create_context(rbc_nb, &rbc);
     create_context(abc_nb, &ctx);  // leaves nb / nxx / nyy / nzz at the ABC grid
@@ -83,7 +88,10 @@ void Inversion::place_shots(fwi_ctx* c, int nbc, int first, int count) {
 void Inversion::compute_gradient() {
     sum_res = 0.0;
     const bool final_check = iteration == max_iteration;  // last pass: misfit only
-    for (long q = 0; q < nPoints; q++) trial[q] = 1.0f / sqrtf(m[q]);
+    if (!upload_updated && !last_trial.empty())
+        trial = last_trial;  // the device model the reference still holds (inversion.cu:206, :324-326)
+    else
+        for (long q = 0; q < nPoints; q++) trial[q] = 1.0f / sqrtf(m[q]);
     check(fwi_set_model_interior(ctx, trial.data()), "fwi_set_model_interior");
     if (!final_check) {
         check(fwi_set_model_interior(rbc, trial.data()), "fwi_set_model_interior");
@@ -158,6 +166,7 @@ double Inversion::linesearch(float alpha) {
         trial[q] = 1.0f / sqrtf(trial[q]);
     }
     sum_res = model_misfit(trial);
+    if (!upload_updated) last_trial = trial;
     return sum_res;
 }
 
@@ -173,6 +182,8 @@ void Inversion::optimization() {
     const float num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2;
     const float den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2;
     step = 0.5f * (num / den);
+    std::cout << "Line search: f(0) = " << f0 << ", f(a1) = " << f1 << ", f(a2) = " << f2 << ", step = " << step
+              << "\n";
 }
 
 // inversion.cu:349-362

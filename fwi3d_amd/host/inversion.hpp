@@ -6,9 +6,10 @@
 // gradient pass.  The misfit and the gradient normalisation are reduced in
 // FP64 / exactly on the GPU.  Deliberate differences from the reference:
 // the updated model is uploaded before the next iteration (inversion.cu:
-// 349-362 never uploads it, so the reference keeps the last line-search
-// trial model), and the illumination is cleared for every gradient
-// (inversion.cu:51 never clears it).
+// 349-362 never uploads it, so the reference computes the next gradient
+// and misfit with the last line-search trial model; the key
+// `fwi_model_upload = last_trial` restores that), and the illumination is
+// cleared for every gradient (inversion.cu:51 never clears it).
 #pragma once
 
 #include "modeling.hpp"
@@ -30,6 +31,8 @@ private:
     std::vector<float> m;         // slowness squared 1/v^2, interior
     std::vector<float> gradient;  // interior, normalised
     std::vector<float> trial;     // interior velocity of a line-search trial
+    std::vector<float> last_trial;  // velocity of the latest trial (fwi_model_upload = last_trial)
+    bool upload_updated = true;     // fwi_model_upload = updated (default) | last_trial
     std::vector<float> obs_data;
     std::vector<float> residual;
 

@@ -89,6 +89,31 @@ def test_inversion_main_lowers_misfit(tiny, tmp_path):
     assert rel_l2(final, w.interior) < rel_l2(init, w.interior)
 
 
+def test_inversion_main_model_upload_modes(tiny, tmp_path):
+    """fwi_model_upload = last_trial reproduces the reference's model
+    handling: the next iteration's misfit is computed with the last
+    line-search trial model, so it equals that trial's misfit f(a2); the
+    default uploads the updated model instead."""
+    import re
+    from scipy.ndimage import gaussian_filter
+    w = tiny
+    init = (1.0 / gaussian_filter(1.0 / w.interior.astype(np.float64), 2.0)).astype(np.float32)
+    f2s, convs = [], []
+    for mode in ("updated", "last_trial"):
+        folder = str(tmp_path / mode)
+        _run("modeling_main", w.write(folder))
+        params = w.write(folder, model=init, kind="init", max_iteration=2, batch_shots=3, depth_mask=20.0)
+        with open(params, "a") as f:
+            f.write("fwi_model_upload = %s\n" % mode)
+        out = _run("inversion_main", params)
+        f2s.append([float(x) for x in re.findall(r"f\(a2\) = ([0-9.eE+-]+),", out)])
+        convs.append(np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_2_iterations.txt")))
+    assert len(f2s[0]) == len(f2s[1]) == 2
+    assert convs[0][0] == convs[1][0] and f2s[0][0] == f2s[1][0]  # same first iteration
+    assert abs(convs[1][1] - f2s[1][0]) <= 1e-5 * f2s[1][0]
+    assert abs(convs[0][1] - f2s[0][0]) > 1e-4 * f2s[0][0]
+
+
 def test_migration_main_backward_corrections(tiny, tmp_path):
     """The backward_corrections key reaches the library: with source
     subtraction and aligned imaging the RTM section changes (and stays
