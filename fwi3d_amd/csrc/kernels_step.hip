@@ -11,7 +11,9 @@
 //       illum += S^2;  image += S R   |  grad += dt R (Dxx + Dzz)(So) vp^2
 //   with the optional backward corrections of StepArgs::phys: the source
 //   term put back into S during reconstruction, So instead of S in the
-//   imaging condition, Dyy in the gradient.
+//   imaging condition, Dyy in the gradient.  They are compiled only into
+//   the CORR instances, so the default backward kernels carry none of
+//   their branches or registers.
 //
 // Work decomposition (gfx950, wave64):
 //  * a block owns a 64 x TY tile of the x-y plane (TY = BY*RPT rows; each of
@@ -136,7 +138,7 @@ __device__ __forceinline__ void bstore(Rsrc r, unsigned voff, unsigned soff, flo
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, AUX);
 }
 
-template <int MODE, int ARITH, int BY, int RPT, int SB>
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
@@ -251,7 +253,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             rb = make_rsrc(a.rold + sbase, vol_b);
         }
         int src_z = -1, src_c = -1;
-        if (!BWD || (a.phys & PHYS_SUBTRACT_SOURCE)) {  // uniform
+        if (!BWD || (CORR && (a.phys & PHYS_SUBTRACT_SOURCE))) {  // uniform
             const long so = a.src_off[shot];
             if (so >= 0) {
                 src_z = (int)(so / L.pp);
@@ -418,7 +420,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     float sn = K * ((dxx + dyy) + dzz);
                     sn = sn + 2.0f * c;
                     sn = sn - o[r];
-                    if ((a.phys & PHYS_SUBTRACT_SOURCE) && z == src_z && cell[r] == src_c) sn = sn + a.inj;
+                    if (CORR && (a.phys & PHYS_SUBTRACT_SOURCE) && z == src_z && cell[r] == src_c) sn = sn + a.inj;
                     float rn = K * ((rxx + ryy) + rzz);
                     rn = rn + 2.0f * cr;
                     rn = rn - ro[r];
@@ -426,14 +428,14 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     res[1][r] = rn;
                     // source field paired with R: the reconstructed one (one
                     // step behind R, the reference) or the current one
-                    const float si = (a.phys & PHYS_ALIGNED_IMAGING) ? c : sn;
+                    const float si = (CORR && (a.phys & PHYS_ALIGNED_IMAGING)) ? c : sn;
                     const float t_ill = si * si;
                     float t_img;
                     if constexpr (MODE == BWD_RTM) {
                         t_img = si * cr;
                     } else {
                         float gterm = a.dt * cr;
-                        gterm = gterm * ((a.phys & PHYS_GRAD_DYY) ? (dxx + dyy) + dzz : dxx + dzz);
+                        gterm = gterm * ((CORR && (a.phys & PHYS_GRAD_DYY)) ? (dxx + dyy) + dzz : dxx + dzz);
                         t_img = gterm * vp2;
                     }
                     if (owner) {
@@ -490,9 +492,9 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     }
 }
 
-template <int MODE, int ARITH, int BY, int RPT, int SB>
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR>
 hipError_t launch_one(const StepArgs& a, dim3 grid, hipStream_t st) {
-    step_kernel<MODE, ARITH, BY, RPT, SB><<<grid, dim3(64, BY * SB), 0, st>>>(a);
+    step_kernel<MODE, ARITH, BY, RPT, SB, CORR><<<grid, dim3(64, BY * SB), 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -500,10 +502,14 @@ using LaunchFn = hipError_t (*)(const StepArgs&, dim3, hipStream_t);
 
 struct Entry {
     int mode, arith, by, rpt, sb;
-    LaunchFn fn;
+    LaunchFn fn, fn_corr;  // fn_corr: the instance with the backward corrections (backward modes)
 };
 
-#define FWI_E(M, A, BY, RPT, SB) {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB>}
+constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
+
+#define FWI_E(M, A, BY, RPT, SB)                                                              \
+    {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false>,                                 \
+     is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M)> : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB)
@@ -553,10 +559,12 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
     const Entry* e = find(mode, arith, t);
     if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
+    const LaunchFn fn = (is_bwd(mode) && a.phys != 0) ? e->fn_corr : e->fn;
+    if (!fn) return hipErrorInvalidConfiguration;
     StepArgs args = a;
     dim3 grid, block;
     step_grid(a.L, t, a.nshots, t.sb, args, grid, block);
-    return e->fn(args, grid, st);
+    return fn(args, grid, st);
 }
 
 }  // namespace fwi

@@ -52,15 +52,23 @@ def rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp):
     rows (xc, yc, zc, r, amplitude); host-side, no device needed."""
     L = load()
     n = ctypes.c_int()
-    rc = L.fwi_rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp, None, 0, ctypes.byref(n))
-    if rc != OK:
-        raise FwiError(rc, "fwi_rbc_blobs")
-    out = np.zeros((n.value, 5), np.float32)
+    # points at least `ratio` apart own disjoint balls of radius ratio/2
+    # inside the box grown by ratio/2: at most 0.7405 of that volume (dense
+    # packing), so one call with this capacity suffices
+    ext = [(k - 1) * dh + ratio for k in (nxx, nyy, nzz)]
+    cap = int(ext[0] * ext[1] * ext[2] * 0.7405 / (np.pi / 6.0 * ratio ** 3)) + 16
+    out = np.zeros((cap, 5), np.float32)
     rc = L.fwi_rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp, out.ctypes.data_as(ctypes.c_void_p),
-                         n.value, ctypes.byref(n))
+                         cap, ctypes.byref(n))
     if rc != OK:
         raise FwiError(rc, "fwi_rbc_blobs")
-    return out
+    if n.value > cap:  # not reached (packing bound); the count is exact, so ask again
+        out = np.zeros((n.value, 5), np.float32)
+        rc = L.fwi_rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp,
+                             out.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n))
+        if rc != OK:
+            raise FwiError(rc, "fwi_rbc_blobs")
+    return out[:n.value].copy()
 
 
 class Grid(ctypes.Structure):

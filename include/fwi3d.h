@@ -111,12 +111,14 @@ int fwi_random_boundary(fwi_ctx* ctx, uint64_t seed, int shot, float amp, float 
 
 /* Reference-style random boundary (random_boundary_bg + random_boundary_gp,
  * modeling.cu:558-666) in one fused launch instead of one launch per blob,
- * deterministic for (seed, shot): boundary cells get per-axis noise
- * (1 - d/nb) * var_vp * n, n ~ N(0,1) (Irwin-Hall of 12 hashed uniforms), then
- * Gaussian blobs at Poisson-disk points (minimum distance `ratio` metres)
- * outside the inner box, radius U[0.2, 1) ratio, amplitude +-U[0.5, 1) var_vp,
- * each truncated at 4 radii, with the clamp to [vmin - var_vp, vmax + var_vp]
- * after each blob, then min(v, vclamp).  Stands in for
+ * deterministic for (seed, shot): each boundary cell starts from the nearest
+ * interior cell and gets per-axis noise (1 - d/nb) * var_vp * n, n ~ N(0,1)
+ * (Irwin-Hall of 12 hashed uniforms), then Gaussian blobs at Poisson-disk
+ * points (minimum distance `ratio` metres) outside the inner box, radius
+ * U[0.2, 1) ratio, amplitude +-U[0.5, 1) var_vp, each truncated at 4 radii,
+ * with the clamp to [vmin - var_vp, vmax + var_vp] (extremes of the interior
+ * model) after every blob, then min(v, vclamp).  Only the nb-cell boundary is
+ * written, so a repeated call starts afresh.  Stands in for
  * Modeling::set_random_boundary(Vp, ratio, varVp) (modeling.cu:558). */
 int fwi_random_boundary_blobs(fwi_ctx* ctx, uint64_t seed, int shot, float ratio, float var_vp, float vclamp);
 
