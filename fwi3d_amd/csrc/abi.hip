@@ -436,7 +436,7 @@ int replicate_model(fwi_ctx* c) {
     return FWI_OK;
 }
 
-const int kTileChoices[][2] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}, {4, 4}, {16, 1}, {2, 2}, {2, 4}};
+const int kTileChoices[][2] = {{4, 1}, {4, 2}, {8, 1}, {8, 2}, {4, 4}, {16, 1}, {16, 2}, {2, 2}, {2, 4}};
 
 // Copy the columns of slot s out of a time-major [nt][nrec_total] matrix
 // into a compact [nt][nrec_s] one (to_slot = false) or back (true).

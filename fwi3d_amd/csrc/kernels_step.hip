@@ -512,7 +512,7 @@ constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M)> : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
-        FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB)
+        FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)
 
 // batched: SB wave groups per block, one shot each (BY * SB <= 16 waves)
 #define FWI_BATCH(M)                                                                                     \

@@ -132,7 +132,7 @@ def test_all_tilings_identical(fl, kind):
     ref = None
     P, Pold, seis = _oracle_forward(case, abc=(kind == 1))
     data = seis.copy()
-    for by, rpt in [(4, 1), (4, 2), (8, 1), (8, 2), (4, 4), (16, 1)]:
+    for by, rpt in [(4, 1), (4, 2), (8, 1), (8, 2), (4, 4), (16, 1), (16, 2)]:
         for zc in (1, 3):
             with case.context(fl.Context, abc=True) as c:
                 c.set_tuning(kind, by, rpt, zc)
