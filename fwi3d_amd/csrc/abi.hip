@@ -35,6 +35,7 @@ struct fwi_ctx {
     int nshots = 1;
     int arith = FWI_ARITH_REFERENCE;
     int phys = 0;                        // FWI_PHYS_* bits of the backward passes
+    bool defer_imaging = true;           // image/illumination updated every other backward launch
     float idh2 = 0, idh3 = 0, dt2 = 0;
 
     float* vp = nullptr;                 // nshots volumes
@@ -270,6 +271,7 @@ int launch_step_timed(fwi_ctx* c, int mode, const Tuning& t, const StepArgs& a) 
 
 StepArgs base_args(fwi_ctx* c) {
     StepArgs a{};
+    a.img = 1;
     a.L = c->L;
     a.sstride = c->L.vol;
     a.nshots = c->nshots;
@@ -325,7 +327,12 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
                                      c->data_tm + (long)(nt - 1 - b0) * c->nrec_total, c->idh3, c->st));
     a.phys = c->phys;
     const int nsteps = nt + c->g.tlag;  // steps of the forward pass being reversed
+    // deferred imaging: pairs of launches share one image/illumination
+    // read-modify-write; a pass (or segment) of odd length ends with a
+    // per-step launch, so no term is left over at its end
+    const bool defer = c->defer_imaging && c->phys == 0 && fwi::step_defer_supported(mode, c->arith, t);
     for (int b = b0; b < b1; ++b) {
+        a.img = !defer ? 1 : ((b - b0) % 2 == 1 ? 2 : (b + 1 < b1 ? 0 : 1));
         a.cur = c->f[1 - c->fcur];
         a.old = c->f[c->fcur];
         a.rcur = c->r[c->rcur];
@@ -1050,6 +1057,12 @@ int fwi_set_physics(fwi_ctx* c, int flags) {
     if (flags & ~(FWI_PHYS_GRAD_DYY | FWI_PHYS_SUBTRACT_SOURCE | FWI_PHYS_ALIGNED_IMAGING))
         return fail(c, FWI_ERR_ARG, "unknown physics flag");
     c->phys = flags;
+    return FWI_OK;
+}
+
+int fwi_set_deferred_imaging(fwi_ctx* c, int enable) {
+    if (!c) return FWI_ERR_ARG;
+    c->defer_imaging = enable != 0;
     return FWI_OK;
 }
 

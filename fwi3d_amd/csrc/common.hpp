@@ -87,6 +87,9 @@ struct StepArgs {
     const long* src_off;   // per shot, device offset of the source cell
     float inj;             // forward: idh3 * wavelet[n+1] (0 when n+1 >= nt)
     int phys;              // backward: Physics bits
+    // backward imaging of this launch: 1 = this step's terms; 0 = deferred
+    // to the next launch; 2 = the deferred terms, then this step's
+    int img;
     // ABC
     const float* b1d;
     int nb;
@@ -100,6 +103,8 @@ struct StepArgs {
 // Host-side launchers (kernels_step.hip).
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st);
 bool step_config_supported(int mode, int arith, const Tuning& t);
+// deferred imaging (StepArgs::img 0/2) available for this backward shape
+bool step_defer_supported(int mode, int arith, const Tuning& t);
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
 
 // Host-side launchers (kernels_aux.hip).  "host layout" buffers here are

@@ -15,6 +15,18 @@
 //   the CORR instances, so the default backward kernels carry none of
 //   their branches or registers.
 //
+// Deferred imaging (IMG).  A backward launch holds, besides this step's
+// fields, the two values the previous step's imaging terms were made of:
+// its new source field is this launch's current one (c) and the receiver
+// field it read is this launch's old receiver value (ro).  So the image and
+// illumination read-modify-write can be done every other launch: IMG = 0
+// leaves them out (the gradient of the FWI mode, whose term needs the
+// previous step's derivatives, is still updated), IMG = 2 adds the previous
+// step's terms (c*ro, c*c) and then this step's, in that order, and IMG = 1
+// is the plain per-step update.  The sums are bit for bit those of per-step
+// updates; the image/illumination traffic halves (RTM: 44 -> 36 B per cell
+// and step on average).
+//
 // Work decomposition (gfx950, wave64):
 //  * a block owns a 64 x TY tile of the x-y plane (TY = BY*RPT rows; each of
 //    the BY waves covers one 64-float row, 256 B, per register row) and
@@ -138,10 +150,14 @@ __device__ __forceinline__ void bstore(Rsrc r, unsigned voff, unsigned soff, flo
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, AUX);
 }
 
-template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR>
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
+    static_assert(IMG == 1 || (BWD && SB == 1 && !CORR), "deferred imaging: single-shot-group backward only");
+    // image (or gradient) and illumination read-modify-writes of this launch
+    constexpr bool RMW_IMG = BWD && (IMG != 0 || MODE == BWD_FWI);
+    constexpr bool RMW_ILL = BWD && IMG != 0;
     constexpr bool ABC = MODE == FWD_ABC;
     constexpr int TY = BY * RPT;
     constexpr int LW = 64 + 8;
@@ -301,8 +317,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                 if (owner) {
 #pragma unroll
                     for (int r = 0; r < RPT; ++r) {
-                        ii[r] = bload<kNT>(ib, co[r], pz);
-                        ll[r] = bload<kNT>(lb, co[r], pz);
+                        if constexpr (RMW_IMG) ii[r] = bload<kNT>(ib, co[r], pz);
+                        if constexpr (RMW_ILL) ll[r] = bload<kNT>(lb, co[r], pz);
                     }
                 }
             }
@@ -340,8 +356,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         img = img + xch[xb][2 * k + 1][ty + r * BY][tx];
                     }
                 }
-                bstore<kNT>(lb, co[r], pz, ill);
-                bstore<kNT>(ib, co[r], pz, img);
+                if constexpr (RMW_ILL) bstore<kNT>(lb, co[r], pz, ill);
+                if constexpr (RMW_IMG) bstore<kNT>(ib, co[r], pz, img);
             }
         };
 
@@ -438,7 +454,17 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         gterm = gterm * ((CORR && (a.phys & PHYS_GRAD_DYY)) ? (dxx + dyy) + dzz : dxx + dzz);
                         t_img = gterm * vp2;
                     }
-                    if (owner) {
+                    if constexpr (IMG == 2) {
+                        // the previous step's terms first: its new source
+                        // field c with the receiver field it read (ro)
+                        ill_r[r] = (il[r] + c * c) + t_ill;
+                        if constexpr (MODE == BWD_RTM)
+                            img_r[r] = (im[r] + c * ro[r]) + t_img;
+                        else
+                            img_r[r] = im[r] + t_img;
+                    } else if constexpr (IMG == 0) {
+                        if constexpr (MODE == BWD_FWI) img_r[r] = im[r] + t_img;
+                    } else if (owner) {
                         ill_r[r] = il[r] + t_ill;
                         img_r[r] = im[r] + t_img;
                     } else if constexpr (SB > 1) {
@@ -492,9 +518,9 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     }
 }
 
-template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR>
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG>
 hipError_t launch_one(const StepArgs& a, dim3 grid, hipStream_t st) {
-    step_kernel<MODE, ARITH, BY, RPT, SB, CORR><<<grid, dim3(64, BY * SB), 0, st>>>(a);
+    step_kernel<MODE, ARITH, BY, RPT, SB, CORR, IMG><<<grid, dim3(64, BY * SB), 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -502,14 +528,21 @@ using LaunchFn = hipError_t (*)(const StepArgs&, dim3, hipStream_t);
 
 struct Entry {
     int mode, arith, by, rpt, sb;
-    LaunchFn fn, fn_corr;  // fn_corr: the instance with the backward corrections (backward modes)
+    LaunchFn fn;       // per-step imaging, no corrections
+    LaunchFn fn_corr;  // with the backward corrections (backward modes)
+    LaunchFn fn_img0;  // deferred imaging: none this launch (single-group backward)
+    LaunchFn fn_img2;  // deferred imaging: previous step's and this step's terms
 };
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
 
-#define FWI_E(M, A, BY, RPT, SB)                                                              \
-    {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false>,                                 \
-     is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M)> : nullptr}
+// backward instances: corrections, and deferred imaging when one shot group
+// fills a block; forward entries repeat the plain instance (never chosen)
+#define FWI_E(M, A, BY, RPT, SB)                                                                         \
+    {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                         \
+     is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                                  \
+     (is_bwd(M) && SB == 1) ? &launch_one<M, A, BY, RPT, SB, false, (is_bwd(M) && SB == 1) ? 0 : 1> : nullptr, \
+     (is_bwd(M) && SB == 1) ? &launch_one<M, A, BY, RPT, SB, false, (is_bwd(M) && SB == 1) ? 2 : 1> : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)
@@ -556,10 +589,19 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     block = dim3(64, t.by * sb);
 }
 
+bool step_defer_supported(int mode, int arith, const Tuning& t) {
+    const Entry* e = find(mode, arith, t);
+    return e && e->fn_img0 && e->fn_img2;
+}
+
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
     const Entry* e = find(mode, arith, t);
     if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
-    const LaunchFn fn = (is_bwd(mode) && a.phys != 0) ? e->fn_corr : e->fn;
+    if (a.img != 1 && (!is_bwd(mode) || a.phys != 0)) return hipErrorInvalidConfiguration;
+    const LaunchFn fn = a.img == 0 ? e->fn_img0
+                        : a.img == 2 ? e->fn_img2
+                        : (is_bwd(mode) && a.phys != 0) ? e->fn_corr
+                                                         : e->fn;
     if (!fn) return hipErrorInvalidConfiguration;
     StepArgs args = a;
     dim3 grid, block;

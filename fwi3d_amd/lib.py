@@ -27,7 +27,7 @@ KIND_FWD_RBC, KIND_FWD_ABC, KIND_BWD_RTM, KIND_BWD_FWI = range(4)
 
 # every symbol declared in include/fwi3d.h
 EXPORTS = [
-    "fwi_create", "fwi_destroy", "fwi_last_error", "fwi_set_arith", "fwi_set_physics", "fwi_set_model", "fwi_set_model_interior",
+    "fwi_create", "fwi_destroy", "fwi_last_error", "fwi_set_arith", "fwi_set_physics", "fwi_set_deferred_imaging", "fwi_set_model", "fwi_set_model_interior",
     "fwi_random_boundary", "fwi_set_wavelet", "fwi_set_dampers", "fwi_set_source", "fwi_set_receivers",
     "fwi_forward", "fwi_forward_steps", "fwi_get_seismogram", "fwi_set_backward_data", "fwi_rtm_backward",
     "fwi_grad_backward", "fwi_backward_steps", "fwi_zero_volume", "fwi_get_volume", "fwi_set_volume",
@@ -96,6 +96,7 @@ def load():
         "fwi_destroy": [P],
         "fwi_set_arith": [P, I],
         "fwi_set_physics": [P, I],
+        "fwi_set_deferred_imaging": [P, I],
         "fwi_set_model": [P, f32],
         "fwi_set_model_interior": [P, f32],
         "fwi_random_boundary": [P, ctypes.c_uint64, I, F, F],
@@ -202,6 +203,11 @@ class Context:
     def set_physics(self, flags):
         """PHYS_* bits of the backward passes (0 = the reference's behaviour)."""
         self._chk(self.L.fwi_set_physics(self.h, flags))
+
+    def set_deferred_imaging(self, enable):
+        """Image/illumination updated every other backward step (default on;
+        bit-identical results) or every step."""
+        self._chk(self.L.fwi_set_deferred_imaging(self.h, 1 if enable else 0))
 
     def set_model(self, vp_pad):
         self._chk(self.L.fwi_set_model(self.h, self._vol(vp_pad)))

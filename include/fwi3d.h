@@ -94,6 +94,14 @@ int fwi_set_arith(fwi_ctx* ctx, int arith);
 #define FWI_PHYS_ALIGNED_IMAGING 4   /* image/illumination from the source field at the receiver field's time */
 int fwi_set_physics(fwi_ctx* ctx, int flags);
 
+/* Backward passes update image and illumination every other time step,
+ * adding two steps' terms in step order (enable = 1, the default, for
+ * single-group launches without corrections; results are bit-identical to
+ * per-step updates) or every step (0).  No reference counterpart: the
+ * reference's cross_correlation (migration.cu:181-202) and build_gradient
+ * (inversion.cu:491-558) update them in every step. */
+int fwi_set_deferred_imaging(fwi_ctx* ctx, int enable);
+
 /* Padded velocity model, host layout (cudaMemcpy of d_Vp, modeling.cu:176).
  * Rejects NaN/Inf and CFL violations. */
 int fwi_set_model(fwi_ctx* ctx, const float* vp_padded);

@@ -89,6 +89,42 @@ def test_backward_bit_exact(fl, fwi):
     assert np.array_equal(g_Ro, Ro)
 
 
+@pytest.mark.parametrize("arith", ["reference", "fp32"])
+@pytest.mark.parametrize("fwi", [False, True])
+def test_deferred_imaging_identical(fl, fwi, arith):
+    """Image/illumination (gradient/illumination) updated every other
+    backward launch, the default, equal per-step updates bit for bit: over a
+    whole pass and over segments of odd and even length (a segment never
+    leaves a term behind); with reference arithmetic both equal the oracle."""
+    case = Case(nt=61)
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(11)
+    data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+    vol = fl.VOL_GRAD if fwi else fl.VOL_IMAGE
+    kind = 1 if fwi else 0
+    outs = []
+    for defer, cuts in ((False, None), (True, None), (True, [0, 3, 8, 9, 20, 44, case.steps])):
+        with case.context(fl.Context) as c:
+            c.set_arith(fl.ARITH_FP32 if arith == "fp32" else fl.ARITH_REFERENCE)
+            c.set_deferred_imaging(defer)
+            c.forward(fl.BOUNDARY_RBC)
+            c.set_backward_data(data)
+            if cuts is None:
+                (c.grad_backward if fwi else c.rtm_backward)()
+            else:
+                for b0, b1 in zip(cuts[:-1], cuts[1:]):
+                    c.backward_steps(kind, b0, b1)
+            outs.append((c.get_volume(vol), c.get_volume(fl.VOL_ILLUM), c.get_volume(fl.VOL_RECV)))
+    assert np.abs(outs[0][0]).max() > 0 and np.abs(outs[0][1]).max() > 0
+    for k in (1, 2):
+        for a, b in zip(outs[0], outs[k]):
+            assert np.array_equal(a, b), k
+    if arith == "reference":
+        _, _, _, _, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=fwi,
+                                              tlag=case.tlag)
+        assert np.array_equal(outs[1][0], image) and np.array_equal(outs[1][1], illum)
+
+
 @pytest.mark.parametrize("phys", [1, 2, 4, 7])
 @pytest.mark.parametrize("fwi", [False, True])
 def test_backward_corrections_bit_exact(fl, fwi, phys):
