@@ -33,16 +33,24 @@ def log(msg):
 
 
 def measured_traffic(kernel_key):
-    """HBM bytes per launch of one step-kernel instance, as measured by the
-    two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, scaled by the copy
-    kernel's calibration) summarised in profiles/*_hbm_traffic.txt; None when
-    no summary covers this kernel instance."""
+    """HBM bytes per launch of one step-kernel shape, as measured by the two
+    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, scaled by the copy
+    kernel's calibration) summarised in profiles/*_hbm_traffic.txt; a
+    backward shape with deferred imaging alternates two instances (_IMG=0,
+    _IMG=2), whose mean is the per-launch figure.  None when no summary
+    covers the shape."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.txt"))):
+        found = {}
         for line in open(path):
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel_key:
-                return int(parts[2]), os.path.relpath(path, ROOT)
+            if len(parts) == 3 and parts[0] == "traffic":
+                found[parts[1]] = int(parts[2])
+        pair = [found.get(kernel_key + "_IMG=%d" % i) for i in (0, 2)]
+        if all(pair):
+            return (pair[0] + pair[1]) // 2, os.path.relpath(path, ROOT)
+        if kernel_key in found:
+            return found[kernel_key], os.path.relpath(path, ROOT)
     return None, None
 
 

@@ -23,9 +23,11 @@
 // leaves them out (the gradient of the FWI mode, whose term needs the
 // previous step's derivatives, is still updated), IMG = 2 adds the previous
 // step's terms (c*ro, c*c) and then this step's, in that order, and IMG = 1
-// is the plain per-step update.  The sums are bit for bit those of per-step
-// updates; the image/illumination traffic halves (RTM: 44 -> 36 B per cell
-// and step on average).
+// is the plain per-step update.  In batched launches the other groups pass
+// both steps' terms to group 0, which adds all groups' terms of the
+// previous step before any of this step's.  The sums are bit for bit those
+// of per-step updates; the image/illumination traffic halves (single shot,
+// RTM: 44 -> 36 B per cell and step on average).
 //
 // Work decomposition (gfx950, wave64):
 //  * a block owns a 64 x TY tile of the x-y plane (TY = BY*RPT rows; each of
@@ -154,7 +156,7 @@ template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
-    static_assert(IMG == 1 || (BWD && SB == 1 && !CORR), "deferred imaging: single-shot-group backward only");
+    static_assert(IMG == 1 || (BWD && !CORR), "deferred imaging: backward launches without corrections");
     // image (or gradient) and illumination read-modify-writes of this launch
     constexpr bool RMW_IMG = BWD && (IMG != 0 || MODE == BWD_FWI);
     constexpr bool RMW_ILL = BWD && IMG != 0;
@@ -166,7 +168,9 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     __shared__ float lds[2][SB * NF][LH][LW];
     // batched backward: the image/illumination terms of groups 1..SB-1,
     // double-buffered by plane parity, summed by group 0
-    constexpr int NX = (BWD && SB > 1) ? 2 * (SB - 1) : 1;
+    // (two terms per group and step; IMG = 2 passes two steps' terms, and
+    // group 0 parks its own terms of this step in the last two slots)
+    constexpr int NX = (BWD && SB > 1) ? (IMG == 2 ? 4 * (SB - 1) + 2 : 2 * (SB - 1)) : 1;
     __shared__ float xch[2][NX][TY][64];
 
     const Layout& L = a.L;
@@ -294,6 +298,9 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
         // illumination with this group's term added (group 0: the running
         // sums, completed with the other groups' terms one plane later)
         float res[NF][RPT], img_r[RPT], ill_r[RPT];
+        // IMG = 2, one group: this step's terms, added after the previous
+        // step's (batched launches keep them in LDS instead)
+        float img_c[RPT], ill_c[RPT];
 
         auto load_plane = [&](int zz, float (&oo)[RPT], float (&vv)[RPT], float (&rr)[RPT], float (&ii)[RPT],
                               float (&ll)[RPT]) {
@@ -349,11 +356,27 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 float ill = ill_r[r], img = img_r[r];
-                if constexpr (SB > 1) {
+                const int row = ty + r * BY;
+                if constexpr (IMG == 2) {
+                    // previous step: the other groups' terms; this step:
+                    // group 0's, then the other groups' (shot order)
+#pragma unroll
+                    for (int k = 0; k < SB - 1; ++k) ill = ill + xch[xb][4 * k][row][tx];
+                    ill = ill + (SB > 1 ? xch[xb][NX - 2][row][tx] : ill_c[r]);
+#pragma unroll
+                    for (int k = 0; k < SB - 1; ++k) ill = ill + xch[xb][4 * k + 2][row][tx];
+                    if constexpr (MODE == BWD_RTM) {
+#pragma unroll
+                        for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][4 * k + 1][row][tx];
+                        img = img + (SB > 1 ? xch[xb][NX - 1][row][tx] : img_c[r]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][4 * k + 3][row][tx];
+                } else if constexpr (SB > 1) {
 #pragma unroll
                     for (int k = 0; k < SB - 1; ++k) {
-                        ill = ill + xch[xb][2 * k][ty + r * BY][tx];
-                        img = img + xch[xb][2 * k + 1][ty + r * BY][tx];
+                        if constexpr (RMW_ILL) ill = ill + xch[xb][2 * k][row][tx];
+                        if constexpr (RMW_IMG) img = img + xch[xb][2 * k + 1][row][tx];
                     }
                 }
                 if constexpr (RMW_ILL) bstore<kNT>(lb, co[r], pz, ill);
@@ -454,22 +477,47 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         gterm = gterm * ((CORR && (a.phys & PHYS_GRAD_DYY)) ? (dxx + dyy) + dzz : dxx + dzz);
                         t_img = gterm * vp2;
                     }
-                    if constexpr (IMG == 2) {
-                        // the previous step's terms first: its new source
-                        // field c with the receiver field it read (ro)
-                        ill_r[r] = (il[r] + c * c) + t_ill;
-                        if constexpr (MODE == BWD_RTM)
-                            img_r[r] = (im[r] + c * ro[r]) + t_img;
-                        else
+                    const int row = ty + r * BY;
+                    if constexpr (IMG == 1) {
+                        if (owner) {
+                            ill_r[r] = il[r] + t_ill;
                             img_r[r] = im[r] + t_img;
+                        } else if constexpr (SB > 1) {
+                            xch[buf][2 * (g - 1)][row][tx] = t_ill;
+                            xch[buf][2 * (g - 1) + 1][row][tx] = t_img;
+                        }
                     } else if constexpr (IMG == 0) {
-                        if constexpr (MODE == BWD_FWI) img_r[r] = im[r] + t_img;
-                    } else if (owner) {
-                        ill_r[r] = il[r] + t_ill;
-                        img_r[r] = im[r] + t_img;
-                    } else if constexpr (SB > 1) {
-                        xch[buf][2 * (g - 1)][ty + r * BY][tx] = t_ill;
-                        xch[buf][2 * (g - 1) + 1][ty + r * BY][tx] = t_img;
+                        if constexpr (MODE == BWD_FWI) {  // the gradient term is not deferred
+                            if (owner)
+                                img_r[r] = im[r] + t_img;
+                            else if constexpr (SB > 1)
+                                xch[buf][2 * (g - 1) + 1][row][tx] = t_img;
+                        }
+                    } else {
+                        // the previous step's terms: its new source field c
+                        // with the receiver field it read (ro)
+                        const float p_ill = c * c;
+                        if (owner) {
+                            ill_r[r] = il[r] + p_ill;
+                            if constexpr (SB > 1)
+                                xch[buf][NX - 2][row][tx] = t_ill;
+                            else
+                                ill_c[r] = t_ill;
+                            if constexpr (MODE == BWD_RTM) {
+                                img_r[r] = im[r] + c * ro[r];
+                                if constexpr (SB > 1)
+                                    xch[buf][NX - 1][row][tx] = t_img;
+                                else
+                                    img_c[r] = t_img;
+                            } else {
+                                img_r[r] = im[r] + t_img;
+                            }
+                        } else if constexpr (SB > 1) {
+                            xch[buf][4 * (g - 1)][row][tx] = p_ill;
+                            xch[buf][4 * (g - 1) + 2][row][tx] = t_ill;
+                            if constexpr (MODE == BWD_RTM) xch[buf][4 * (g - 1) + 1][row][tx] = c * ro[r];
+                            xch[buf][4 * (g - 1) + 3][row][tx] = t_img;
+                        }
                     }
                 }
             }
@@ -536,13 +584,16 @@ struct Entry {
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
 
-// backward instances: corrections, and deferred imaging when one shot group
-// fills a block; forward entries repeat the plain instance (never chosen)
-#define FWI_E(M, A, BY, RPT, SB)                                                                         \
-    {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                         \
-     is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                                  \
-     (is_bwd(M) && SB == 1) ? &launch_one<M, A, BY, RPT, SB, false, (is_bwd(M) && SB == 1) ? 0 : 1> : nullptr, \
-     (is_bwd(M) && SB == 1) ? &launch_one<M, A, BY, RPT, SB, false, (is_bwd(M) && SB == 1) ? 2 : 1> : nullptr}
+// backward instances: with the corrections, and the two deferred-imaging
+// ones (RTM, and FWI with one shot group: batched FWI gains little, its
+// gradient being updated every step, and would spill registers); other
+// entries name the plain instance there (never chosen)
+constexpr bool defers(int m, int sb) { return m == BWD_RTM || (m == BWD_FWI && sb == 1); }
+#define FWI_E(M, A, BY, RPT, SB)                                                                      \
+    {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
+     is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
+     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 0 : 1> : nullptr,           \
+     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 2 : 1> : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)

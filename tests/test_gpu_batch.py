@@ -166,6 +166,48 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
     assert np.array_equal(img_b, want_img)
 
 
+@pytest.mark.parametrize("sb", [2, 4])
+def test_batched_deferred_imaging_identical(fl, sb):
+    """Batched RTM with image and illumination updated every other launch
+    (the default; the other groups hand group 0 two steps' terms) equals
+    per-step updates bit for bit, over a whole pass and over the same
+    segments of odd and even length (each segment ends by folding the shot
+    groups' partial sums into group 0, so segmented and whole passes round
+    differently when there are several groups)."""
+    case = Case(nx=40, ny=36, nz=28, nt=51, amp=0.0)
+    S = 4
+    shots = _shots(case, S)
+    outs = []
+    cuts_odd = [0, 5, 6, 17, 30, case.steps]
+    for defer, cuts in ((False, None), (True, None), (False, cuts_odd), (True, cuts_odd)):
+        c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+        c.set_model_interior(case.interior)
+        for s, (src, rec) in enumerate(shots):
+            c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+            c.set_source(*src, slot=s)
+            c.set_receivers(rec, slot=s)
+        c.set_wavelet(case.wavelet)
+        for kind in (0, 2):
+            c.set_shots_per_block(kind, sb)
+        c.set_deferred_imaging(defer)
+        c.forward(fl.BOUNDARY_RBC)
+        rng = np.random.default_rng(sb)
+        for s in range(S):
+            d = (c.seismogram(slot=s) + rng.normal(0, 1e-3, (len(shots[s][1]), case.nt))).astype(np.float32)
+            c.set_backward_data(d, slot=s)
+        if cuts is None:
+            c.rtm_backward()
+        else:
+            for b0, b1 in zip(cuts[:-1], cuts[1:]):
+                c.backward_steps(0, b0, b1)
+        outs.append((c.get_volume(fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)))
+        c.close()
+    assert np.abs(outs[0][0]).max() > 0
+    for k in (1, 3):
+        assert np.array_equal(outs[k][0], outs[k - 1][0]), k
+        assert np.array_equal(outs[k][1], outs[k - 1][1]), k
+
+
 def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
     """FWI gradient mode with all backward corrections on, two shots per
     block, and a shot with no receivers (its receiver field stays zero, its
