@@ -275,6 +275,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pass-steps", type=int, default=0,
                     help="profiling only: cut each pass to this many time steps")
+    ap.add_argument("--probe-shapes", default="",
+                    help="profiling only (with --pass-steps): also run that many forward and RTM steps with each "
+                         "of these single-shot shapes, e.g. '16,1,1 8,1,1' (BY, RPT, z-chunks), so that one "
+                         "counter pass covers every shape the autotuner may pick")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profiling only: one 1 GiB copy-kernel launch (counter calibration)")
     ap.add_argument("--tuning", default="",
@@ -404,6 +408,15 @@ def main():
     pct_hbm = value * BYTES_PER_UPDATE / HBM_PEAK_GBS
     # the timed shot's padded model (the per-shot leg below replaces its boundary)
     ctx_vp = ctx.get_volume(lib.VOL_MODEL) if not args.no_cpu_baseline else None
+
+    if args.pass_steps and args.probe_shapes:
+        for shape in args.probe_shapes.split():
+            by, rpt, zc = [int(v) for v in shape.split(",")]
+            for kind in (lib.KIND_FWD_RBC, lib.KIND_BWD_RTM):
+                ctx.set_tuning(kind, by, rpt, zc)
+            ctx.forward_steps(lib.BOUNDARY_RBC, 0, args.pass_steps)
+            ctx.backward_steps(0, 0, args.pass_steps)
+            log("probe shape %s done" % shape)
 
     aux = None
     if not args.no_aux and not args.pass_steps:
