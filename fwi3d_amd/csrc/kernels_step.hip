@@ -152,6 +152,11 @@ __device__ __forceinline__ void bstore(Rsrc r, unsigned voff, unsigned soff, flo
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, AUX);
 }
 
+template <int P>
+struct Parity {
+    static constexpr int value = P;
+};
+
 template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
@@ -292,8 +297,10 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
         float qn[NF][RPT];
         float hx[NF][RPT];
         float hy[NF][HY];
-        float o[RPT], v[RPT], ro[RPT], im[RPT], il[RPT];
-        float o_n[RPT], v_n[RPT], ro_n[RPT], im_n[RPT], il_n[RPT];
+        // (two sets, swapped by plane parity: the loop body is instantiated
+        // once per parity, so the swap costs no moves)
+        float o0[RPT], v0[RPT], ro0[RPT], im0[RPT], il0[RPT];
+        float o1[RPT], v1[RPT], ro1[RPT], im1[RPT], il1[RPT];
         // results of the plane just computed; img_r/ill_r: image and
         // illumination with this group's term added (group 0: the running
         // sums, completed with the other groups' terms one plane later)
@@ -391,12 +398,29 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             for (int d = 0; d < 9; ++d)
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) q[f][r][d] = bload(fb[f], qo[r], (unsigned)(zb - 4 + d) * plane_b);
-        load_plane(zb, o, v, ro, im, il);
+        load_plane(zb, o0, v0, ro0, im0, il0);
         stage_plane(0, 4);
         __syncthreads();
 
-        for (int z = zb; z < ze; ++z) {
-            const int buf = (z - zb) & 1;
+        // One plane.  Single-group launches pass the parity (z - zb) & 1 as
+        // a compile-time argument (Parity<0/1>): it selects the LDS buffer
+        // and which stream set is current, so no moves or address updates
+        // are needed between planes.  Batched launches (Parity<2>) keep a
+        // runtime buffer index and move the next stream set into place:
+        // the doubled loop body would not fit their 128-VGPR budget.
+        auto plane = [&](auto par, const int z, const int dyn_buf) {
+            constexpr int P = decltype(par)::value;
+            const int buf = P == 2 ? dyn_buf : P;
+            float(&o)[RPT] = P == 1 ? o1 : o0;
+            float(&v)[RPT] = P == 1 ? v1 : v0;
+            float(&ro)[RPT] = P == 1 ? ro1 : ro0;
+            float(&im)[RPT] = P == 1 ? im1 : im0;
+            float(&il)[RPT] = P == 1 ? il1 : il0;
+            float(&o_n)[RPT] = P == 1 ? o0 : o1;
+            float(&v_n)[RPT] = P == 1 ? v0 : v1;
+            float(&ro_n)[RPT] = P == 1 ? ro0 : ro1;
+            float(&im_n)[RPT] = P == 1 ? im0 : im1;
+            float(&il_n)[RPT] = P == 1 ? il0 : il1;
             // 1. prefetch plane z+1: queue head z+5, halos, streams.  The
             // loop body is one straight block (no branch on the last plane,
             // whose prefetched values are simply unused; the queue head is
@@ -546,17 +570,29 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     for (int d = 0; d < 8; ++d) q[f][r][d] = q[f][r][d + 1];
                     q[f][r][8] = qn[f][r];
                 }
+            __syncthreads();
+        };
+        if constexpr (SB == 1) {
+            int z = zb;
+            for (; z + 1 < ze; z += 2) {
+                plane(Parity<0>{}, z, 0);
+                plane(Parity<1>{}, z + 1, 1);
+            }
+            if (z < ze) plane(Parity<0>{}, z, 0);
+        } else {
+            for (int z = zb; z < ze; ++z) {
+                plane(Parity<2>{}, z, (z - zb) & 1);
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                o[r] = o_n[r];
-                v[r] = v_n[r];
-                if constexpr (BWD) {
-                    ro[r] = ro_n[r];
-                    im[r] = im_n[r];
-                    il[r] = il_n[r];
+                for (int r = 0; r < RPT; ++r) {
+                    o0[r] = o1[r];
+                    v0[r] = v1[r];
+                    if constexpr (BWD) {
+                        ro0[r] = ro1[r];
+                        im0[r] = im1[r];
+                        il0[r] = il1[r];
+                    }
                 }
             }
-            __syncthreads();
         }
         // batched backward: complete the last plane (its terms were
         // exchanged before the loop's last barrier)
