@@ -1116,6 +1116,10 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     }
     const bool bwd = kind >= 2;
     if (bwd) CHKRC(ensure_backward_buffers(c, kind - 2));
+    // backward trials time an even number of steps: with deferred imaging a
+    // pass alternates a launch without and one with the image update, and an
+    // odd count would add a per-step update that whole passes do not have
+    if (bwd) steps += steps & 1;
     const Tuning saved = c->tune[kind];
     const bool saved_graph = c->use_graph, saved_kt = c->ktime;
     c->use_graph = false;

@@ -402,12 +402,13 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
         stage_plane(0, 4);
         __syncthreads();
 
-        // One plane.  Single-group launches pass the parity (z - zb) & 1 as
-        // a compile-time argument (Parity<0/1>): it selects the LDS buffer
-        // and which stream set is current, so no moves or address updates
-        // are needed between planes.  Batched launches (Parity<2>) keep a
-        // runtime buffer index and move the next stream set into place:
-        // the doubled loop body would not fit their 128-VGPR budget.
+        // One plane.  The parity (z - zb) & 1 is passed as a compile-time
+        // argument (Parity<0/1>): it selects the LDS buffer and which stream
+        // set is current, so no moves or address updates are needed between
+        // planes.  Blocks of 1024 threads with several shot groups, and
+        // groups of four rows per thread, keep a runtime buffer index and
+        // move the next stream set into place (Parity<2>): the doubled loop
+        // body would not fit their register budget.
         auto plane = [&](auto par, const int z, const int dyn_buf) {
             constexpr int P = decltype(par)::value;
             const int buf = P == 2 ? dyn_buf : P;
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                 }
             __syncthreads();
         };
-        if constexpr (SB == 1) {
+        if constexpr (SB == 1 || (BY * SB <= 8 && RPT <= 2)) {
             int z = zb;
             for (; z + 1 < ze; z += 2) {
                 plane(Parity<0>{}, z, 0);

@@ -236,8 +236,9 @@ int fwi_get_shots_per_block(fwi_ctx* ctx, int kind, int* sb);
 
 /* Autotune tile shape, z-chunking and shots per block for a pass kind
  * (0 forward RBC, 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing
- * `steps` steps of each candidate; cache_path (may be NULL) persists the
- * choice.  The trial steps run on the context's buffers, which are zeroed
+ * `steps` steps of each candidate (backward kinds: rounded up to an even
+ * count, the deferred-imaging pattern of whole passes); cache_path (may be
+ * NULL) persists the choice.  The trial steps run on the context's buffers, which are zeroed
  * afterwards: autotune before setting up a pass. */
 int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);
 
