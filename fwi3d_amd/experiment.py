@@ -18,6 +18,13 @@ Both use the same kernels and the same batched contexts; wall times count
 every forward/backward pass, line search, network inference and fallback.
 
     python -m fwi3d_amd.experiment --scale small --out results.json
+
+The "full" scale is the 401 x 401 x 201 configuration itself (10 m, 50-cell
+boundary, 3000 samples at 0.8 ms, 60 Hz, initial model smoothed with
+sigma = 150 m); its network is trained on 201 x 201 x 101 models with the
+same dt and frequency.  It runs in two phases so that each fits one GPU
+session: `--phase train --save-net net.npy`, then `--phase fwi --load-net
+net.npy`.
 """
 import argparse
 import json
@@ -37,6 +44,9 @@ SCALES = {
                   patch=(32, 32, 32), sigma=8.0),
     "medium": dict(grid=(160, 160, 80), nb=30, dt=1e-3, nt=1100, fmax=30.0, n_train=48, train_iters=3, steps=900,
                    patch=(32, 32, 32), sigma=12.0),
+    "full": dict(grid=(401, 401, 201), nb=50, dt=8e-4, nt=3000, fmax=60.0, n_train=32, train_iters=3, steps=900,
+                 patch=(32, 32, 32), sigma=15.0, train_grid=(201, 201, 101), train_nb=30, train_nt=1500,
+                 max_learned=6),
 }
 
 
@@ -68,27 +78,42 @@ def problem(v_true, nb, dt, nt, fmax, nshots_side=4, seed=0):
     return w, learned.observed(w, v_true)
 
 
-def run(scale="small", seed=0, log=None):
+def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net=""):
+    """phase "all": train, then compare plain and learned FWI; "train": train
+    only (save_net: where to store the weights, a float32 .npy); "fwi": the
+    comparison with the weights of load_net."""
     p = SCALES[scale]
     nx, ny, nz = p["grid"]
+    tnx, tny, tnz = p.get("train_grid", p["grid"])
     log = log or (lambda m: None)
     out = {"scale": scale, "grid": [nx, ny, nz], "nb": p["nb"], "dt": p["dt"], "nt": p["nt"], "fmax": p["fmax"],
-           "shots": 16}
+           "shots": 16, "train_grid": [tnx, tny, tnz]}
 
-    # 1. training triples from plain FWI on seeded random models
-    t0 = time.perf_counter()
-    triples = learned.make_triples(p["n_train"], nx, ny, nz, p["train_iters"], p["nt"], dt=p["dt"], fmax=p["fmax"],
-                                   seed0=1000 + 1000 * seed, sigma=p["sigma"], nb=p["nb"], log=log)
-    out["training_triples"] = len(triples)
-    out["triples_seconds"] = round(time.perf_counter() - t0, 2)
-
-    # 2. train the network on the GPU
     from .unet import UNet
     net = UNet(seed=seed + 7)
-    t0 = time.perf_counter()
-    losses = learned.train(net, triples, p["steps"], patch=p["patch"], batch=4, lr=1e-3, seed=seed, log=log)
-    out["train_seconds"] = round(time.perf_counter() - t0, 2)
-    out["train_loss_first_last"] = [float(np.mean(losses[:20])), float(np.mean(losses[-20:]))]
+    if phase in ("all", "train"):
+        # 1. training triples from plain FWI on seeded random models
+        t0 = time.perf_counter()
+        triples = learned.make_triples(p["n_train"], tnx, tny, tnz, p["train_iters"], p.get("train_nt", p["nt"]),
+                                       dt=p["dt"], fmax=p["fmax"], seed0=1000 + 1000 * seed, sigma=p["sigma"],
+                                       nb=p.get("train_nb", p["nb"]), log=log)
+        out["training_triples"] = len(triples)
+        out["triples_seconds"] = round(time.perf_counter() - t0, 2)
+
+        # 2. train the network on the GPU
+        t0 = time.perf_counter()
+        losses = learned.train(net, triples, p["steps"], patch=p["patch"], batch=4, lr=1e-3, seed=seed, log=log)
+        out["train_seconds"] = round(time.perf_counter() - t0, 2)
+        out["train_loss_first_last"] = [float(np.mean(losses[:20])), float(np.mean(losses[-20:]))]
+        del triples
+        if save_net:
+            np.save(save_net, np.asarray(net.params, np.float32))
+        if phase == "train":
+            net.close()
+            return out
+    else:
+        net.params = np.load(load_net).astype(np.float32)
+        out["network"] = "loaded"
 
     # 3. the test problem
     v_true = true_model(nx, ny, nz, seed=seed + 1)
@@ -104,7 +129,8 @@ def run(scale="small", seed=0, log=None):
     out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
                     "initial_model_rel_l2": model_err(v0)}
     target = hist_plain[-1]
-    v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=target, max_iterations=10, log=log)
+    v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=target,
+                                                   max_iterations=p.get("max_learned", 10), log=log)
     out["learned"] = {"misfit": hist_l, "seconds": round(sec_l, 3), "model_rel_l2": model_err(v_l),
                       "updates": used, "reached_target": bool(hist_l[-1] <= target)}
     out["time_to_misfit_ratio"] = round(sec_l / sec_plain, 4) if hist_l[-1] <= target else None
@@ -118,8 +144,14 @@ def main(argv=None):
     ap.add_argument("--scale", choices=sorted(SCALES), default="small")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
+    ap.add_argument("--phase", choices=["all", "train", "fwi"], default="all")
+    ap.add_argument("--save-net", default="", help="phase train/all: store the trained weights (.npy)")
+    ap.add_argument("--load-net", default="", help="phase fwi: weights to use (.npy)")
     a = ap.parse_args(argv)
-    res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if a.phase == "fwi" and not a.load_net:
+        ap.error("--phase fwi needs --load-net")
+    res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True), phase=a.phase,
+              save_net=a.save_net, load_net=a.load_net)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:
