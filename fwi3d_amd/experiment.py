@@ -78,10 +78,12 @@ def problem(v_true, nb, dt, nt, fmax, nshots_side=4, seed=0):
     return w, learned.observed(w, v_true)
 
 
-def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net=""):
+def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", n_train=None, train_offset=0):
     """phase "all": train, then compare plain and learned FWI; "train": train
-    only (save_net: where to store the weights, a float32 .npy); "fwi": the
-    comparison with the weights of load_net."""
+    only (save_net: where to store the weights, a float32 .npy; with
+    load_net, training continues from those weights, so a large training set
+    can be worked through in parts: n_train models starting at model
+    train_offset); "fwi": the comparison with the weights of load_net."""
     p = SCALES[scale]
     nx, ny, nz = p["grid"]
     tnx, tny, tnz = p.get("train_grid", p["grid"])
@@ -91,12 +93,17 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net=""):
 
     from .unet import UNet
     net = UNet(seed=seed + 7)
+    if load_net:
+        net.params = np.load(load_net).astype(np.float32)
+        out["network"] = "loaded"
     if phase in ("all", "train"):
         # 1. training triples from plain FWI on seeded random models
         t0 = time.perf_counter()
-        triples = learned.make_triples(p["n_train"], tnx, tny, tnz, p["train_iters"], p.get("train_nt", p["nt"]),
-                                       dt=p["dt"], fmax=p["fmax"], seed0=1000 + 1000 * seed, sigma=p["sigma"],
-                                       nb=p.get("train_nb", p["nb"]), log=log)
+        n = p["n_train"] if n_train is None else n_train
+        out["training_models"] = [train_offset, train_offset + n]
+        triples = learned.make_triples(n, tnx, tny, tnz, p["train_iters"], p.get("train_nt", p["nt"]),
+                                       dt=p["dt"], fmax=p["fmax"], seed0=1000 + 1000 * seed + train_offset,
+                                       sigma=p["sigma"], nb=p.get("train_nb", p["nb"]), log=log)
         out["training_triples"] = len(triples)
         out["triples_seconds"] = round(time.perf_counter() - t0, 2)
 
@@ -111,9 +118,6 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net=""):
         if phase == "train":
             net.close()
             return out
-    else:
-        net.params = np.load(load_net).astype(np.float32)
-        out["network"] = "loaded"
 
     # 3. the test problem
     v_true = true_model(nx, ny, nz, seed=seed + 1)
@@ -146,12 +150,14 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--phase", choices=["all", "train", "fwi"], default="all")
     ap.add_argument("--save-net", default="", help="phase train/all: store the trained weights (.npy)")
-    ap.add_argument("--load-net", default="", help="phase fwi: weights to use (.npy)")
+    ap.add_argument("--load-net", default="", help="weights to use (phase fwi) or to continue training from (.npy)")
+    ap.add_argument("--n-train", type=int, default=None, help="training models (default: the scale's)")
+    ap.add_argument("--train-offset", type=int, default=0, help="index of the first training model")
     a = ap.parse_args(argv)
     if a.phase == "fwi" and not a.load_net:
         ap.error("--phase fwi needs --load-net")
     res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True), phase=a.phase,
-              save_net=a.save_net, load_net=a.load_net)
+              save_net=a.save_net, load_net=a.load_net, n_train=a.n_train, train_offset=a.train_offset)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:
