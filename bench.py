@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--batch-tuning", default="",
                     help="fixed batched launch shapes instead of autotuning, e.g. 'fwd=8,2,16,1;rtm=4,2,1,4' "
                          "(BY, RPT, z-chunks, shots per block)")
-    ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--batch-graph", type=int, default=0, choices=[0, 1],
                     help="capture the batched passes' time loops in hipGraphs (1) or launch on the stream (0)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the blob-boundary and dataset-tool timings")
