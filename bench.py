@@ -147,6 +147,22 @@ def blob_boundary(w, ctx, lib, shots=3, ratio=100.0, var_vp=1000.0):
             "reference_launch_pattern_hbm_floor_ms": round(1e3 * floor, 1)}
 
 
+def corrections_cost(ctx, lib, steps=40):
+    """RTM backward launch time with the reference's physics and with all
+    three optional corrections (d2/dy2, source subtraction, aligned
+    imaging), over the same steps of the single-shot context."""
+    out = {}
+    for name, phys in (("reference", 0), ("all_corrections", 7)):
+        ctx.set_physics(phys)
+        ctx.backward_steps(0, 1000, 1000 + steps)
+        ms, n = ctx.last_kernel_timing()
+        out["rtm_launch_us_" + name] = round(ms / max(n, 1) * 1e3, 1)
+    ctx.set_physics(0)
+    log("RTM launch: %.1f us (reference physics), %.1f us (all corrections)"
+        % (out["rtm_launch_us_reference"], out["rtm_launch_us_all_corrections"]))
+    return out
+
+
 def dataset_tools(w):
     """The C++ initial-model filter (setup_main) against scipy's
     gaussian_filter, which the reference's run/generate_setup.py:64-66 calls,
@@ -421,6 +437,8 @@ def main():
     aux = None
     if not args.no_aux and not args.pass_steps:
         aux = {"blob_random_boundary": blob_boundary(w, ctx, lib), "dataset_tools": dataset_tools(w)}
+        if not args.graph:
+            aux["backward_corrections"] = corrections_cost(ctx, lib)
 
     batched = None
     if args.batch_shots > 0:
