@@ -147,7 +147,7 @@ def blob_boundary(w, ctx, lib, shots=3, ratio=100.0, var_vp=1000.0):
             "reference_launch_pattern_hbm_floor_ms": round(1e3 * floor, 1)}
 
 
-def corrections_cost(ctx, lib, steps=40):
+def corrections_cost(ctx, lib, steps=200):
     """RTM backward launch time with the reference's physics and with all
     three optional corrections (d2/dy2, source subtraction, aligned
     imaging), over the same steps of the single-shot context."""
