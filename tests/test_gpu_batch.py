@@ -173,8 +173,10 @@ def test_batched_deferred_imaging_identical(fl, sb):
     per-step updates bit for bit, over a whole pass and over the same
     segments of odd and even length (each segment ends by folding the shot
     groups' partial sums into group 0, so segmented and whole passes round
-    differently when there are several groups)."""
-    case = Case(nx=40, ny=36, nz=28, nt=51, amp=0.0)
+    differently when there are several groups).  An odd number of planes
+    and two z-chunks (of 18 and 17 planes) exercise the last-plane path of
+    the parity-unrolled loop."""
+    case = Case(nx=40, ny=36, nz=27, nt=51, amp=0.0)
     S = 4
     shots = _shots(case, S)
     outs = []
@@ -189,6 +191,8 @@ def test_batched_deferred_imaging_identical(fl, sb):
         c.set_wavelet(case.wavelet)
         for kind in (0, 2):
             c.set_shots_per_block(kind, sb)
+            by, rpt, _ = c.tuning(kind)
+            c.set_tuning(kind, by, rpt, 2)
         c.set_deferred_imaging(defer)
         c.forward(fl.BOUNDARY_RBC)
         rng = np.random.default_rng(sb)

@@ -95,8 +95,10 @@ def test_deferred_imaging_identical(fl, fwi, arith):
     """Image/illumination (gradient/illumination) updated every other
     backward launch, the default, equal per-step updates bit for bit: over a
     whole pass and over segments of odd and even length (a segment never
-    leaves a term behind); with reference arithmetic both equal the oracle."""
-    case = Case(nt=61)
+    leaves a term behind); with reference arithmetic both equal the oracle.
+    An odd number of planes in two z-chunks (20 and 19) exercises the
+    last-plane path of the parity-unrolled loop."""
+    case = Case(nz=31, nt=61)
     P, Pold, seis = _oracle_forward(case)
     rng = np.random.default_rng(11)
     data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
@@ -106,6 +108,7 @@ def test_deferred_imaging_identical(fl, fwi, arith):
     for defer, cuts in ((False, None), (True, None), (True, [0, 3, 8, 9, 20, 44, case.steps])):
         with case.context(fl.Context) as c:
             c.set_arith(fl.ARITH_FP32 if arith == "fp32" else fl.ARITH_REFERENCE)
+            c.set_tuning(2 + kind, 4, 2, 2)
             c.set_deferred_imaging(defer)
             c.forward(fl.BOUNDARY_RBC)
             c.set_backward_data(data)
