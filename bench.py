@@ -163,6 +163,25 @@ def corrections_cost(ctx, lib, steps=200):
     return out
 
 
+def post_processing(ctx, w, reps=3):
+    """Whole-call times (PCIe transfers included) of the per-run and
+    per-shot host-facing steps the reference does in host loops: image
+    export (migration.cu:90-117; the gradient export of inversion.cu:142-169
+    runs the same kernel) and the residual with its FP64 misfit
+    (inversion.cu:190-201) of one shot's 3000 x 40401 seismogram."""
+    obs = ctx.seismogram()
+    out = {}
+    for name, fn in (("export_image_ms", lambda: ctx.export_image()),
+                     ("residual_misfit_ms", lambda: ctx.residual(obs))):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        out[name] = round((time.perf_counter() - t0) / reps * 1e3, 1)
+    log("post-processing: %s" % out)
+    return out
+
+
 def dataset_tools(w):
     """The C++ initial-model filter (setup_main) against scipy's
     gaussian_filter, which the reference's run/generate_setup.py:64-66 calls,
@@ -439,6 +458,7 @@ def main():
         aux = {"blob_random_boundary": blob_boundary(w, ctx, lib), "dataset_tools": dataset_tools(w)}
         if not args.graph:
             aux["backward_corrections"] = corrections_cost(ctx, lib)
+        aux["post_processing"] = post_processing(ctx, w)
 
     batched = None
     if args.batch_shots > 0:
