@@ -330,7 +330,7 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     // deferred imaging: pairs of launches share one image/illumination
     // read-modify-write; a pass (or segment) of odd length ends with a
     // per-step launch, so no term is left over at its end
-    const bool defer = c->defer_imaging && c->phys == 0 && fwi::step_defer_supported(mode, c->arith, t);
+    const bool defer = c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
     for (int b = b0; b < b1; ++b) {
         a.img = !defer ? 1 : ((b - b0) % 2 == 1 ? 2 : (b + 1 < b1 ? 0 : 1));
         a.cur = c->f[1 - c->fcur];

@@ -103,8 +103,9 @@ struct StepArgs {
 // Host-side launchers (kernels_step.hip).
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st);
 bool step_config_supported(int mode, int arith, const Tuning& t);
-// deferred imaging (StepArgs::img 0/2) available for this backward shape
-bool step_defer_supported(int mode, int arith, const Tuning& t);
+// deferred imaging (StepArgs::img 0/2) available for this backward shape,
+// with or without the corrections of StepArgs::phys
+bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections);
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
 
 // Host-side launchers (kernels_aux.hip).  "host layout" buffers here are

@@ -13,7 +13,9 @@
 //   term put back into S during reconstruction, So instead of S in the
 //   imaging condition, Dyy in the gradient.  They are compiled only into
 //   the CORR instances, so the default backward kernels carry none of
-//   their branches or registers.
+//   their branches or registers.  With aligned imaging the previous step's
+//   deferred terms (below) use its current source field, this launch's
+//   old value, instead of its new one.
 //
 // Deferred imaging (IMG).  A backward launch holds, besides this step's
 // fields, the two values the previous step's imaging terms were made of:
@@ -161,7 +163,7 @@ template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
-    static_assert(IMG == 1 || (BWD && !CORR), "deferred imaging: backward launches without corrections");
+    static_assert(IMG == 1 || (BWD && (!CORR || SB == 1)), "deferred imaging: backward launches only");
     // image (or gradient) and illumination read-modify-writes of this launch
     constexpr bool RMW_IMG = BWD && (IMG != 0 || MODE == BWD_FWI);
     constexpr bool RMW_ILL = BWD && IMG != 0;
@@ -520,8 +522,10 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         }
                     } else {
                         // the previous step's terms: its new source field c
-                        // with the receiver field it read (ro)
-                        const float p_ill = c * c;
+                        // (aligned imaging: its current one, this launch's old
+                        // value o) with the receiver field it read (ro)
+                        const float ps = (CORR && (a.phys & PHYS_ALIGNED_IMAGING)) ? o[r] : c;
+                        const float p_ill = ps * ps;
                         if (owner) {
                             ill_r[r] = il[r] + p_ill;
                             if constexpr (SB > 1)
@@ -529,7 +533,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                             else
                                 ill_c[r] = t_ill;
                             if constexpr (MODE == BWD_RTM) {
-                                img_r[r] = im[r] + c * ro[r];
+                                img_r[r] = im[r] + ps * ro[r];
                                 if constexpr (SB > 1)
                                     xch[buf][NX - 1][row][tx] = t_img;
                                 else
@@ -540,7 +544,7 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         } else if constexpr (SB > 1) {
                             xch[buf][4 * (g - 1)][row][tx] = p_ill;
                             xch[buf][4 * (g - 1) + 2][row][tx] = t_ill;
-                            if constexpr (MODE == BWD_RTM) xch[buf][4 * (g - 1) + 1][row][tx] = c * ro[r];
+                            if constexpr (MODE == BWD_RTM) xch[buf][4 * (g - 1) + 1][row][tx] = ps * ro[r];
                             xch[buf][4 * (g - 1) + 3][row][tx] = t_img;
                         }
                     }
@@ -615,22 +619,29 @@ struct Entry {
     int mode, arith, by, rpt, sb;
     LaunchFn fn;       // per-step imaging, no corrections
     LaunchFn fn_corr;  // with the backward corrections (backward modes)
-    LaunchFn fn_img0;  // deferred imaging: none this launch (single-group backward)
+    LaunchFn fn_img0;  // deferred imaging: none this launch
     LaunchFn fn_img2;  // deferred imaging: previous step's and this step's terms
+    LaunchFn fn_corr_img0, fn_corr_img2;  // the same with the corrections (one shot group per block)
 };
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
 
-// backward instances: with the corrections, and the two deferred-imaging
-// ones (RTM, and FWI with one shot group: batched FWI gains little, its
-// gradient being updated every step, and would spill registers); other
-// entries name the plain instance there (never chosen)
+// backward instances: with the corrections, the two deferred-imaging ones
+// (RTM, and FWI with one shot group: batched FWI gains little, its gradient
+// being updated every step, and would spill registers), and those two with
+// the corrections for one shot group per block; other entries name the
+// plain instance there (never chosen)
 constexpr bool defers(int m, int sb) { return m == BWD_RTM || (m == BWD_FWI && sb == 1); }
+constexpr bool defers_corr(int m, int sb) { return is_bwd(m) && sb == 1; }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
     {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
      defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 0 : 1> : nullptr,           \
-     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 2 : 1> : nullptr}
+     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 2 : 1> : nullptr,           \
+     defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 0 : 1> \
+                        : nullptr,                                                                     \
+     defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 2 : 1> \
+                        : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)
@@ -677,19 +688,21 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     block = dim3(64, t.by * sb);
 }
 
-bool step_defer_supported(int mode, int arith, const Tuning& t) {
+bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections) {
     const Entry* e = find(mode, arith, t);
-    return e && e->fn_img0 && e->fn_img2;
+    if (!e) return false;
+    return corrections ? (e->fn_corr_img0 && e->fn_corr_img2) : (e->fn_img0 && e->fn_img2);
 }
 
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
     const Entry* e = find(mode, arith, t);
     if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
-    if (a.img != 1 && (!is_bwd(mode) || a.phys != 0)) return hipErrorInvalidConfiguration;
-    const LaunchFn fn = a.img == 0 ? e->fn_img0
-                        : a.img == 2 ? e->fn_img2
-                        : (is_bwd(mode) && a.phys != 0) ? e->fn_corr
-                                                         : e->fn;
+    if (a.img != 1 && !is_bwd(mode)) return hipErrorInvalidConfiguration;
+    const bool corr = is_bwd(mode) && a.phys != 0;
+    const LaunchFn fn = a.img == 0   ? (corr ? e->fn_corr_img0 : e->fn_img0)
+                        : a.img == 2 ? (corr ? e->fn_corr_img2 : e->fn_img2)
+                        : corr       ? e->fn_corr
+                                     : e->fn;
     if (!fn) return hipErrorInvalidConfiguration;
     StepArgs args = a;
     dim3 grid, block;

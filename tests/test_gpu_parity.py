@@ -89,13 +89,15 @@ def test_backward_bit_exact(fl, fwi):
     assert np.array_equal(g_Ro, Ro)
 
 
+@pytest.mark.parametrize("phys", [0, 7])
 @pytest.mark.parametrize("arith", ["reference", "fp32"])
 @pytest.mark.parametrize("fwi", [False, True])
-def test_deferred_imaging_identical(fl, fwi, arith):
+def test_deferred_imaging_identical(fl, fwi, arith, phys):
     """Image/illumination (gradient/illumination) updated every other
     backward launch, the default, equal per-step updates bit for bit: over a
     whole pass and over segments of odd and even length (a segment never
-    leaves a term behind); with reference arithmetic both equal the oracle.
+    leaves a term behind), with and without the three corrections; with
+    reference arithmetic both equal the oracle.
     An odd number of planes in two z-chunks (20 and 19) exercises the
     last-plane path of the parity-unrolled loop."""
     case = Case(nz=31, nt=61)
@@ -109,6 +111,7 @@ def test_deferred_imaging_identical(fl, fwi, arith):
         with case.context(fl.Context) as c:
             c.set_arith(fl.ARITH_FP32 if arith == "fp32" else fl.ARITH_REFERENCE)
             c.set_tuning(2 + kind, 4, 2, 2)
+            c.set_physics(phys)
             c.set_deferred_imaging(defer)
             c.forward(fl.BOUNDARY_RBC)
             c.set_backward_data(data)
@@ -124,7 +127,7 @@ def test_deferred_imaging_identical(fl, fwi, arith):
             assert np.array_equal(a, b), k
     if arith == "reference":
         _, _, _, _, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=fwi,
-                                              tlag=case.tlag)
+                                              tlag=case.tlag, phys=phys, wavelet_=case.wavelet, src=case.src)
         assert np.array_equal(outs[1][0], image) and np.array_equal(outs[1][1], illum)
 
 
