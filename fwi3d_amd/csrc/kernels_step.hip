@@ -627,17 +627,19 @@ struct Entry {
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
 
 // backward instances: with the corrections, the two deferred-imaging ones
-// (RTM, and FWI with one shot group: batched FWI gains little, its gradient
-// being updated every step, and would spill registers), and those two with
-// the corrections for one shot group per block; other entries name the
-// plain instance there (never chosen)
-constexpr bool defers(int m, int sb) { return m == BWD_RTM || (m == BWD_FWI && sb == 1); }
+// (RTM, and FWI in blocks of up to 512 threads: in larger batched blocks
+// the FWI variants would spill registers), and those two with the
+// corrections for one shot group per block; other entries name the plain
+// instance there (never chosen)
+constexpr bool defers(int m, int by, int sb) {
+    return m == BWD_RTM || (m == BWD_FWI && (sb == 1 || by * sb <= 8));
+}
 constexpr bool defers_corr(int m, int sb) { return is_bwd(m) && sb == 1; }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
     {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
-     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 0 : 1> : nullptr,           \
-     defers(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, SB) ? 2 : 1> : nullptr,           \
+     defers(M, BY, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, BY, SB) ? 0 : 1> : nullptr,   \
+     defers(M, BY, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, BY, SB) ? 2 : 1> : nullptr,   \
      defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 0 : 1> \
                         : nullptr,                                                                     \
      defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 2 : 1> \

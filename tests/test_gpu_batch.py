@@ -166,9 +166,10 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
     assert np.array_equal(img_b, want_img)
 
 
+@pytest.mark.parametrize("fwi", [False, True])
 @pytest.mark.parametrize("sb", [2, 4])
-def test_batched_deferred_imaging_identical(fl, sb):
-    """Batched RTM with image and illumination updated every other launch
+def test_batched_deferred_imaging_identical(fl, sb, fwi):
+    """Batched RTM (FWI) with image (gradient) and illumination updated every other launch
     (the default; the other groups hand group 0 two steps' terms) equals
     per-step updates bit for bit, over a whole pass and over the same
     segments of odd and even length (each segment ends by folding the shot
@@ -189,7 +190,7 @@ def test_batched_deferred_imaging_identical(fl, sb):
             c.set_source(*src, slot=s)
             c.set_receivers(rec, slot=s)
         c.set_wavelet(case.wavelet)
-        for kind in (0, 2):
+        for kind in (0, 3 if fwi else 2):
             c.set_shots_per_block(kind, sb)
             by, rpt, _ = c.tuning(kind)
             c.set_tuning(kind, by, rpt, 2)
@@ -200,11 +201,11 @@ def test_batched_deferred_imaging_identical(fl, sb):
             d = (c.seismogram(slot=s) + rng.normal(0, 1e-3, (len(shots[s][1]), case.nt))).astype(np.float32)
             c.set_backward_data(d, slot=s)
         if cuts is None:
-            c.rtm_backward()
+            (c.grad_backward if fwi else c.rtm_backward)()
         else:
             for b0, b1 in zip(cuts[:-1], cuts[1:]):
-                c.backward_steps(0, b0, b1)
-        outs.append((c.get_volume(fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)))
+                c.backward_steps(1 if fwi else 0, b0, b1)
+        outs.append((c.get_volume(fl.VOL_GRAD if fwi else fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)))
         c.close()
     assert np.abs(outs[0][0]).max() > 0
     for k in (1, 3):
