@@ -169,8 +169,10 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
 @pytest.mark.parametrize("fwi", [False, True])
 @pytest.mark.parametrize("sb", [2, 4])
 def test_batched_deferred_imaging_identical(fl, sb, fwi):
-    """Batched RTM (FWI) with image (gradient) and illumination updated every other launch
-    (the default; the other groups hand group 0 two steps' terms) equals
+    """Batched RTM with image and illumination updated every other launch
+    (the default; the other groups hand group 0 two steps' terms), and
+    batched FWI with the illumination deferred (blocks of up to 512 threads;
+    larger ones update every step), equal
     per-step updates bit for bit, over a whole pass and over the same
     segments of odd and even length (each segment ends by folding the shot
     groups' partial sums into group 0, so segmented and whole passes round
