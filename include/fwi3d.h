@@ -96,10 +96,10 @@ int fwi_set_physics(fwi_ctx* ctx, int flags);
 
 /* Backward passes update image and illumination every other time step,
  * adding two steps' terms in step order (enable = 1, the default; results
- * are bit-identical to per-step updates) or every step (0).  Batched FWI
- * passes, and batched passes with corrections, always update every step;
- * the FWI gradient term is always added per step, only its illumination
- * deferred.  No reference counterpart: the
+ * are bit-identical to per-step updates) or every step (0).  Batched
+ * passes with corrections, and batched FWI passes in blocks of more than
+ * 512 threads, always update every step; the FWI gradient term is always
+ * added per step, only its illumination deferred.  No reference counterpart: the
  * reference's cross_correlation (migration.cu:181-202) and build_gradient
  * (inversion.cu:491-558) update them in every step. */
 int fwi_set_deferred_imaging(fwi_ctx* ctx, int enable);
