@@ -163,6 +163,24 @@ def corrections_cost(ctx, lib, steps=200):
     return out
 
 
+def abc_forward_cost(ctx, w, lib, steps=200):
+    """Forward launch time with the Cerjan absorbing boundary (the FWI
+    passes that compute calculated data and line-search misfits,
+    inversion.cu:178-188,315-347) against the random-boundary forward, over
+    the same steps of the single-shot context (tuned for the latter)."""
+    from fwi3d_amd import fwi
+    ctx.set_dampers(fwi.b1d_table(w.nb, 0.004))
+    ctx.autotune(lib.KIND_FWD_ABC, 4)
+    out = {"abc_tuning": list(ctx.tuning(lib.KIND_FWD_ABC))}
+    for name, boundary in (("rbc", lib.BOUNDARY_RBC), ("abc", lib.BOUNDARY_ABC)):
+        ctx.forward_steps(boundary, 0, steps)
+        ms, n = ctx.last_kernel_timing()
+        out["forward_launch_us_" + name] = round(ms / max(n, 1) * 1e3, 1)
+    log("forward launch: %.1f us (random boundary), %.1f us (absorbing boundary)"
+        % (out["forward_launch_us_rbc"], out["forward_launch_us_abc"]))
+    return out
+
+
 def post_processing(ctx, w, reps=3):
     """Whole-call times (PCIe transfers included) of the per-run and
     per-shot host-facing steps the reference does in host loops: image
@@ -458,6 +476,7 @@ def main():
         aux = {"blob_random_boundary": blob_boundary(w, ctx, lib), "dataset_tools": dataset_tools(w)}
         if not args.graph:
             aux["backward_corrections"] = corrections_cost(ctx, lib)
+            aux["absorbing_boundary"] = abc_forward_cost(ctx, w, lib)
         aux["post_processing"] = post_processing(ctx, w)
 
     batched = None

@@ -128,6 +128,30 @@ __device__ __forceinline__ float damper(DampAxis i, DampAxis j, DampAxis k) {
     return ((t1 + t2) + t3) - 5.0f;
 }
 
+// A thread's x-y part of damper(z, x, y), fixed while it marches along z:
+// the whole damper when the plane is outside the z layer (dxy), and what
+// the three cases with the z axis on need (the same float operations as
+// damper() with i = z, j = x, k = y).
+struct DampXY {
+    float dxy;  // damper(off, x, y)
+    float e;    // one of x, y on: that axis' b
+    float t2;   // both on: b_x + b_y
+    float by;
+    int cnt;    // axes on among x, y
+};
+
+__device__ __forceinline__ DampXY damp_xy(DampAxis x, DampAxis y) {
+    return {damper(DampAxis{0.0f, false}, x, y), y.on ? y.b : x.b, x.b + y.b, y.b, (int)x.on + (int)y.on};
+}
+
+// damper(z, x, y) for a plane inside the z layer (z.on)
+__device__ __forceinline__ float damper_zon(float bz, float bx, const DampXY& d) {
+    if (d.cnt == 0) return bz;
+    if (d.cnt == 1) return (bz + d.e) - 1.0f;
+    const float t1 = bz + bx, t3 = bz + d.by;
+    return ((t1 + d.t2) + t3) - 5.0f;
+}
+
 // Buffer loads and stores.  Every volume of a launch is addressed through a
 // wave-uniform buffer resource (base, size in bytes) held in scalar
 // registers; a lane supplies one 32-bit in-plane byte offset and the plane
@@ -261,11 +285,18 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             hyo[m] = (h < 8 && xld && yy < L.ny) ? (unsigned)(x + yy * L.px) * 4u : OOB;
         }
 
-        DampAxis dmp_x{0.0f, false}, dmp_y[RPT];
+        DampAxis dmp_x{0.0f, false};
+        DampXY dmp_xy[RPT];
+        // block-uniform: does the tile reach into the absorbing layer in x
+        // or y?  If not, its cells are damped only on planes within nb of a
+        // z face; elsewhere the damper is exactly 1 and is skipped.
+        bool tile_damped = false;
         if constexpr (ABC) {
             dmp_x = damp_axis(a.b1d, a.nb, x, L.nx);
 #pragma unroll
-            for (int r = 0; r < RPT; ++r) dmp_y[r] = damp_axis(a.b1d, a.nb, y0 + ty + r * BY, L.ny);
+            for (int r = 0; r < RPT; ++r) dmp_xy[r] = damp_xy(dmp_x, damp_axis(a.b1d, a.nb, y0 + ty + r * BY, L.ny));
+            const int xa = tile_x * 64, ya = y0;
+            tile_damped = xa < a.nb || xa + 63 >= L.nx - a.nb || ya < a.nb || ya + TY - 1 >= L.ny - a.nb;
         }
 
         // this group's shot: buffer resources (wave-uniform)
@@ -445,7 +476,11 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
 
             // 2. compute plane z (results kept in registers)
             DampAxis dmp_z{0.0f, false};
-            if constexpr (ABC) dmp_z = damp_axis(a.b1d, a.nb, z, L.nz);
+            bool damp = false;  // block-uniform
+            if constexpr (ABC) {
+                dmp_z = damp_axis(a.b1d, a.nb, z, L.nz);
+                damp = tile_damped || dmp_z.on;
+            }
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 const int lr = ty + r * BY + 4;
@@ -465,13 +500,17 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     float oo = o[r];
                     float dmp = 1.0f;
                     if constexpr (ABC) {
-                        dmp = damper(dmp_z, dmp_x, dmp_y[r]);
-                        oo = oo * dmp;
+                        if (damp) {
+                            dmp = dmp_z.on ? damper_zon(dmp_z.b, dmp_x.b, dmp_xy[r]) : dmp_xy[r].dxy;
+                            oo = oo * dmp;
+                        }
                     }
                     float nv = K * ((dxx + dyy) + dzz);
                     nv = nv + 2.0f * c;
                     nv = nv - oo;
-                    if constexpr (ABC) nv = nv * dmp;
+                    if constexpr (ABC) {
+                        if (damp) nv = nv * dmp;
+                    }
                     if (z == src_z && cell[r] == src_c) nv = nv + a.inj;
                     res[0][r] = nv;
                 } else {
