@@ -79,9 +79,24 @@ def cpu_baseline(w, vp_pad, seconds):
     t_bwd = time.perf_counter() - t0
     updates = cells * (nf + 2 * nbk)
     rate = updates / (t_fwd + t_bwd) / 1e9
+    # the per-run / per-shot host steps the reference runs in host loops:
+    # image export (migration.cu:90-117) and residual + misfit of one shot
+    # (inversion.cu:190-201), on this workload's sizes
+    rng = np.random.default_rng(0)
+    img = rng.normal(0, 1, vp_pad.shape).astype(np.float32)
+    ill = rng.uniform(0.5, 2.0, vp_pad.shape).astype(np.float32)
+    t0 = time.perf_counter()
+    O.export_seismic(img, ill, w.nb, w.dh)
+    t_exp = time.perf_counter() - t0
+    obs = rng.normal(0, 1, (len(w.rec), w.nt)).astype(np.float32)
+    cal = rng.normal(0, 1, obs.shape).astype(np.float32)
+    t0 = time.perf_counter()
+    O.misfit(obs, cal)
+    t_mis = time.perf_counter() - t0
     return {"value": round(rate, 4), "unit": "Gcell-upd/s", "cores": threads, "kind": "port",
             "sample": "%d forward + %d RTM backward steps on the full 501x501x301 padded grid (%.1f s)"
-                      % (nf, nbk, t_fwd + t_bwd)}
+                      % (nf, nbk, t_fwd + t_bwd),
+            "export_image_ms": round(t_exp * 1e3, 1), "residual_misfit_ms": round(t_mis * 1e3, 1)}
 
 
 def cpu_vs_gpu_two_layer(lib, configs):
