@@ -35,7 +35,7 @@ struct fwi_ctx {
     int nshots = 1;
     int arith = FWI_ARITH_REFERENCE;
     int phys = 0;                        // FWI_PHYS_* bits of the backward passes
-    bool defer_imaging = true;           // image/illumination updated every other backward launch
+    bool defer_imaging = true;           // image/illumination updated every second or third backward launch
     float idh2 = 0, idh3 = 0, dt2 = 0;
 
     float* vp = nullptr;                 // nshots volumes
@@ -328,11 +328,22 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     a.phys = c->phys;
     const int nsteps = nt + c->g.tlag;  // steps of the forward pass being reversed
     // deferred imaging: pairs of launches share one image/illumination
-    // read-modify-write; a pass (or segment) of odd length ends with a
-    // per-step launch, so no term is left over at its end
+    // read-modify-write, and a pass (or segment) of odd length ends with a
+    // per-step launch; for one shot group per block without corrections the
+    // image is updated every other launch and the illumination every third,
+    // and the terms still pending at the end are added by one elementwise
+    // launch.  Either way no term is left over when the pass ends.
     const bool defer = c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
+    const bool defer3 = defer && fwi::step_defer3_supported(mode, c->arith, t, c->phys != 0);
     for (int b = b0; b < b1; ++b) {
-        a.img = !defer ? 1 : ((b - b0) % 2 == 1 ? 2 : (b + 1 < b1 ? 0 : 1));
+        const int i = b - b0;
+        if (defer3) {
+            const bool img_now = mode == fwi::BWD_RTM && i % 2 == 1;
+            const bool ill_now = i % 3 == 2;
+            a.img = img_now ? (ill_now ? 5 : 3) : (ill_now ? 4 : 0);
+        } else {
+            a.img = !defer ? 1 : (i % 2 == 1 ? 2 : (b + 1 < b1 ? 0 : 1));
+        }
         a.cur = c->f[1 - c->fcur];
         a.old = c->f[c->fcur];
         a.rcur = c->r[c->rcur];
@@ -347,6 +358,16 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
         if (b + 1 < b1 && b + 1 < nt)
             HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
                                          c->data_tm + (long)(nt - 2 - b) * c->nrec_total, c->idh3, c->st));
+    }
+    if (defer3 && b0 < b1) {
+        // the terms of the steps after the last update: the illumination's
+        // of the last one or two steps, the image's of the last step when
+        // the segment has odd length
+        const int last = b1 - b0 - 1;
+        const int ill_terms = (last % 3 + 1) % 3;
+        const int img_term = mode == fwi::BWD_RTM && last % 2 == 0;
+        HIPCHK(c, fwi::launch_flush_imaging(c->f[1 - c->fcur], c->f[c->fcur], c->r[1 - c->rcur], a.image, a.illum,
+                                            c->L, c->nshots, c->L.vol, ill_terms, img_term, c->st));
     }
     const int groups = c->nshots / t.sb;
     if (b0 < b1 && groups > 1) {
@@ -1116,10 +1137,11 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     }
     const bool bwd = kind >= 2;
     if (bwd) CHKRC(ensure_backward_buffers(c, kind - 2));
-    // backward trials time an even number of steps: with deferred imaging a
-    // pass alternates a launch without and one with the image update, and an
-    // odd count would add a per-step update that whole passes do not have
-    if (bwd) steps += steps & 1;
+    // backward trials time a multiple of six steps: with deferred imaging a
+    // pass repeats the image update every other launch and the illumination
+    // update every second or third, and another count would add updates at
+    // the end that whole passes have only once
+    if (bwd) steps = (steps + 5) / 6 * 6;
     const Tuning saved = c->tune[kind];
     const bool saved_graph = c->use_graph, saved_kt = c->ktime;
     c->use_graph = false;

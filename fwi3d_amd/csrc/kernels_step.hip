@@ -30,6 +30,14 @@
 // previous step before any of this step's.  The sums are bit for bit those
 // of per-step updates; the image/illumination traffic halves (single shot,
 // RTM: 44 -> 36 B per cell and step on average).
+// One shot group per block and no corrections: the illumination term of
+// the step before the previous one is known as well (its new source field
+// is this launch's old value o), so the illumination is updated every third
+// launch and the image every other one: IMG = 3 adds the image terms of the
+// previous step and this one, IMG = 4 the illumination terms of the two
+// previous steps (o*o, c*c) and this one, IMG = 5 both (RTM: 36 -> 34.7 B
+// per cell and step on average).  Terms still pending when a pass or a
+// segment ends are added by a separate elementwise launch.
 //
 // Work decomposition (gfx950, wave64):
 //  * a block owns a 64 x TY tile of the x-y plane (TY = BY*RPT rows; each of
@@ -188,9 +196,11 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
     static_assert(IMG == 1 || (BWD && (!CORR || SB == 1)), "deferred imaging: backward launches only");
+    static_assert(IMG <= 2 || (SB == 1 && !CORR && (IMG == 4 || MODE == BWD_RTM)),
+                  "image every other launch, illumination every third: one shot group, no corrections");
     // image (or gradient) and illumination read-modify-writes of this launch
-    constexpr bool RMW_IMG = BWD && (IMG != 0 || MODE == BWD_FWI);
-    constexpr bool RMW_ILL = BWD && IMG != 0;
+    constexpr bool RMW_IMG = BWD && (IMG == 1 || IMG == 2 || IMG == 3 || IMG == 5 || MODE == BWD_FWI);
+    constexpr bool RMW_ILL = BWD && (IMG == 1 || IMG == 2 || IMG == 4 || IMG == 5);
     constexpr bool ABC = MODE == FWD_ABC;
     constexpr int TY = BY * RPT;
     constexpr int LW = 64 + 8;
@@ -412,6 +422,10 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     }
 #pragma unroll
                     for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][4 * k + 3][row][tx];
+                } else if constexpr (IMG >= 3) {
+                    // one group: this step's deferred terms, after the earlier ones
+                    if constexpr (RMW_ILL) ill = ill + ill_c[r];
+                    if constexpr (MODE == BWD_RTM && RMW_IMG) img = img + img_c[r];
                 } else if constexpr (SB > 1) {
 #pragma unroll
                     for (int k = 0; k < SB - 1; ++k) {
@@ -559,6 +573,24 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                             else if constexpr (SB > 1)
                                 xch[buf][2 * (g - 1) + 1][row][tx] = t_img;
                         }
+                    } else if constexpr (IMG >= 3) {
+                        // one group, no corrections: the illumination terms
+                        // of the two previous steps (new source fields o and
+                        // c), the image term of the previous one (c with the
+                        // receiver field it read, ro); this step's are added
+                        // after them in finish_plane
+                        if constexpr (RMW_ILL) {
+                            ill_r[r] = (il[r] + o[r] * o[r]) + c * c;
+                            ill_c[r] = t_ill;
+                        }
+                        if constexpr (MODE == BWD_RTM) {
+                            if constexpr (RMW_IMG) {
+                                img_r[r] = im[r] + c * ro[r];
+                                img_c[r] = t_img;
+                            }
+                        } else {
+                            img_r[r] = im[r] + t_img;
+                        }
                     } else {
                         // the previous step's terms: its new source field c
                         // (aligned imaging: its current one, this launch's old
@@ -661,6 +693,9 @@ struct Entry {
     LaunchFn fn_img0;  // deferred imaging: none this launch
     LaunchFn fn_img2;  // deferred imaging: previous step's and this step's terms
     LaunchFn fn_corr_img0, fn_corr_img2;  // the same with the corrections (one shot group per block)
+    // one shot group, no corrections: image every other launch, illumination
+    // every third (IMG 3: image, 4: illumination, 5: both; 4 also for FWI)
+    LaunchFn fn_img3, fn_img4, fn_img5;
 };
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
@@ -674,6 +709,8 @@ constexpr bool defers(int m, int by, int sb) {
     return m == BWD_RTM || (m == BWD_FWI && (sb == 1 || by * sb <= 8));
 }
 constexpr bool defers_corr(int m, int sb) { return is_bwd(m) && sb == 1; }
+constexpr bool cad3_ill(int m, int sb) { return is_bwd(m) && sb == 1; }
+constexpr bool cad3_img(int m, int sb) { return m == BWD_RTM && sb == 1; }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
     {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
@@ -682,7 +719,10 @@ constexpr bool defers_corr(int m, int sb) { return is_bwd(m) && sb == 1; }
      defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 0 : 1> \
                         : nullptr,                                                                     \
      defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 2 : 1> \
-                        : nullptr}
+                        : nullptr,                                                                     \
+     cad3_img(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, SB) ? 3 : 1> : nullptr,         \
+     cad3_ill(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_ill(M, SB) ? 4 : 1> : nullptr,         \
+     cad3_img(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, SB) ? 5 : 1> : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)
@@ -735,15 +775,27 @@ bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections
     return corrections ? (e->fn_corr_img0 && e->fn_corr_img2) : (e->fn_img0 && e->fn_img2);
 }
 
+bool step_defer3_supported(int mode, int arith, const Tuning& t, bool corrections) {
+    const Entry* e = find(mode, arith, t);
+    if (!e || corrections || !e->fn_img0 || !e->fn_img4) return false;
+    return mode == BWD_FWI || (e->fn_img3 && e->fn_img5);
+}
+
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
     const Entry* e = find(mode, arith, t);
     if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
     if (a.img != 1 && !is_bwd(mode)) return hipErrorInvalidConfiguration;
+    if (a.img < 0 || a.img > 5) return hipErrorInvalidConfiguration;
     const bool corr = is_bwd(mode) && a.phys != 0;
-    const LaunchFn fn = a.img == 0   ? (corr ? e->fn_corr_img0 : e->fn_img0)
-                        : a.img == 2 ? (corr ? e->fn_corr_img2 : e->fn_img2)
-                        : corr       ? e->fn_corr
-                                     : e->fn;
+    LaunchFn fn = nullptr;
+    switch (a.img) {
+        case 0: fn = corr ? e->fn_corr_img0 : e->fn_img0; break;
+        case 1: fn = corr ? e->fn_corr : e->fn; break;
+        case 2: fn = corr ? e->fn_corr_img2 : e->fn_img2; break;
+        case 3: fn = corr ? nullptr : e->fn_img3; break;
+        case 4: fn = corr ? nullptr : e->fn_img4; break;
+        default: fn = corr ? nullptr : e->fn_img5; break;
+    }
     if (!fn) return hipErrorInvalidConfiguration;
     StepArgs args = a;
     dim3 grid, block;

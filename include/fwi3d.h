@@ -96,7 +96,10 @@ int fwi_set_physics(fwi_ctx* ctx, int flags);
 
 /* Backward passes update image and illumination every other time step,
  * adding two steps' terms in step order (enable = 1, the default; results
- * are bit-identical to per-step updates) or every step (0).  Batched
+ * are bit-identical to per-step updates) or every step (0).  With one shot
+ * per block and no corrections the illumination is updated every third
+ * step (three terms in step order), and terms still pending when a pass or
+ * a segment of backward_steps ends are added by one extra launch.  Batched
  * passes with corrections, and batched FWI passes in blocks of more than
  * 512 threads, always update every step; the FWI gradient term is always
  * added per step, only its illumination deferred.  No reference counterpart: the
@@ -237,8 +240,8 @@ int fwi_get_shots_per_block(fwi_ctx* ctx, int kind, int* sb);
 
 /* Autotune tile shape, z-chunking and shots per block for a pass kind
  * (0 forward RBC, 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing
- * `steps` steps of each candidate (backward kinds: rounded up to an even
- * count, the deferred-imaging pattern of whole passes); cache_path (may be
+ * `steps` steps of each candidate (backward kinds: rounded up to a multiple
+ * of six, the deferred-imaging pattern of whole passes); cache_path (may be
  * NULL) persists the choice.  The trial steps run on the context's buffers, which are zeroed
  * afterwards: autotune before setting up a pass. */
 int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);

@@ -93,10 +93,10 @@ def test_backward_bit_exact(fl, fwi):
 @pytest.mark.parametrize("arith", ["reference", "fp32"])
 @pytest.mark.parametrize("fwi", [False, True])
 def test_deferred_imaging_identical(fl, fwi, arith, phys):
-    """Image/illumination (gradient/illumination) updated every other
-    backward launch, the default, equal per-step updates bit for bit: over a
-    whole pass and over segments of odd and even length (a segment never
-    leaves a term behind), with and without the three corrections; with
+    """Image/illumination (gradient/illumination) updated every second or
+    third backward launch, the default, equal per-step updates bit for bit:
+    over a whole pass and over segments of every length modulo 6 (a segment
+    never leaves a term behind), with and without the three corrections; with
     reference arithmetic both equal the oracle.
     An odd number of planes in two z-chunks (20 and 19) exercises the
     last-plane path of the parity-unrolled loop."""
@@ -107,7 +107,7 @@ def test_deferred_imaging_identical(fl, fwi, arith, phys):
     vol = fl.VOL_GRAD if fwi else fl.VOL_IMAGE
     kind = 1 if fwi else 0
     outs = []
-    for defer, cuts in ((False, None), (True, None), (True, [0, 3, 8, 9, 20, 44, case.steps])):
+    for defer, cuts in ((False, None), (True, None), (True, [0, 3, 8, 9, 11, 15, 20, 44, case.steps])):
         with case.context(fl.Context) as c:
             c.set_arith(fl.ARITH_FP32 if arith == "fp32" else fl.ARITH_REFERENCE)
             c.set_tuning(2 + kind, 4, 2, 2)
