@@ -329,10 +329,10 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     const int nsteps = nt + c->g.tlag;  // steps of the forward pass being reversed
     // deferred imaging: pairs of launches share one image/illumination
     // read-modify-write, and a pass (or segment) of odd length ends with a
-    // per-step launch; for one shot group per block without corrections the
-    // image is updated every other launch and the illumination every third,
-    // and the terms still pending at the end are added by one elementwise
-    // launch.  Either way no term is left over when the pass ends.
+    // per-step launch; without corrections the image is updated every other
+    // launch and the illumination every third, and the terms still pending
+    // at the end are added by one elementwise launch.  Either way no term is
+    // left over when the pass ends.
     const bool defer = c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
     const bool defer3 = defer && fwi::step_defer3_supported(mode, c->arith, t, c->phys != 0);
     for (int b = b0; b < b1; ++b) {
@@ -367,7 +367,7 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
         const int ill_terms = (last % 3 + 1) % 3;
         const int img_term = mode == fwi::BWD_RTM && last % 2 == 0;
         HIPCHK(c, fwi::launch_flush_imaging(c->f[1 - c->fcur], c->f[c->fcur], c->r[1 - c->rcur], a.image, a.illum,
-                                            c->L, c->nshots, c->L.vol, ill_terms, img_term, c->st));
+                                            c->L, c->nshots, t.sb, c->L.vol, ill_terms, img_term, c->st));
     }
     const int groups = c->nshots / t.sb;
     if (b0 < b1 && groups > 1) {

@@ -89,9 +89,9 @@ struct StepArgs {
     int phys;              // backward: Physics bits
     // backward imaging of this launch: 1 = this step's terms; 0 = deferred
     // to a later launch; 2 = the previous step's terms, then this step's;
-    // one shot group, no corrections: 3 = image of the previous step and
-    // this one, 4 = illumination of the two previous steps and this one,
-    // 5 = both (other terms deferred)
+    // without corrections: 3 = image of the previous step and this one,
+    // 4 = illumination of the two previous steps and this one, 5 = both
+    // (other terms deferred)
     int img;
     // ABC
     const float* b1d;
@@ -110,7 +110,7 @@ bool step_config_supported(int mode, int arith, const Tuning& t);
 // with or without the corrections of StepArgs::phys
 bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections);
 // the image every other launch and the illumination every third (StepArgs::img
-// 0, 3, 4, 5) available for this backward shape
+// 0, 3, 4, 5) available for this backward shape (no corrections)
 bool step_defer3_supported(int mode, int arith, const Tuning& t, bool corrections);
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
 
@@ -128,7 +128,7 @@ hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, f
 hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st);
 hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st);
 hipError_t launch_flush_imaging(const float* cur, const float* old, const float* rold, float* image, float* illum,
-                                const Layout& L, int nshots, long istride, int ill_terms, int img_term,
+                                const Layout& L, int nshots, int sb, long istride, int ill_terms, int img_term,
                                 hipStream_t st);
 hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec,
                          int ncells, const float* data_row, float idh3, hipStream_t st);

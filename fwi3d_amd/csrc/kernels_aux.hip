@@ -297,16 +297,17 @@ __global__ void __launch_bounds__(256) fold_groups_kernel(float* __restrict__ v,
 // image every other one).  cur, old and rold are the fields the next launch
 // would read: the newest reconstructed source field (c), the one before it
 // (o) and the receiver field the last launch read (ro).  Pending terms are
-// added in step order: illum += o*o (two pending), illum += c*c (one or
-// two), image += c*ro (RTM).  Updated cells only (4 <= x, y, z < n-4); shot
-// s's fields at s*vol, its image and illumination at s*istride.
+// added in step order, each step's in shot order within the shot group:
+// illum += o*o (two pending), illum += c*c (one or two), image += c*ro
+// (RTM).  Updated cells only (4 <= x, y, z < n-4); shot s's fields at
+// s*vol; group grp (shots grp*sb .. grp*sb+sb-1) accumulates at grp*istride.
 __global__ void __launch_bounds__(256) flush_imaging_kernel(const float* __restrict__ cur,
                                                             const float* __restrict__ old,
                                                             const float* __restrict__ rold, float* __restrict__ image,
-                                                            float* __restrict__ illum, Layout L, long istride,
+                                                            float* __restrict__ illum, Layout L, int sb, long istride,
                                                             int ill_terms, int img_term) {
     const int z = 4 + blockIdx.y;
-    const long fb = (long)blockIdx.z * L.vol + (long)z * L.pp;
+    const long fb = (long)blockIdx.z * sb * L.vol + (long)z * L.pp;
     const long ib = (long)blockIdx.z * istride + (long)z * L.pp;
     const int n = (L.ny - 8) * L.px;
     for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
@@ -314,17 +315,24 @@ __global__ void __launch_bounds__(256) flush_imaging_kernel(const float* __restr
         const int x = q - (y - 4) * L.px;
         if (x < 4 || x >= L.nx - 4) continue;
         const long e = (long)y * L.px + x;
-        const float c = cur[fb + e];
         if (ill_terms > 0) {
             float il = illum[ib + e];
-            if (ill_terms > 1) {
-                const float o = old[fb + e];
-                il = il + o * o;
+            if (ill_terms > 1)
+                for (int g = 0; g < sb; ++g) {
+                    const float o = old[fb + g * L.vol + e];
+                    il = il + o * o;
+                }
+            for (int g = 0; g < sb; ++g) {
+                const float c = cur[fb + g * L.vol + e];
+                il = il + c * c;
             }
-            il = il + c * c;
             illum[ib + e] = il;
         }
-        if (img_term) image[ib + e] = image[ib + e] + c * rold[fb + e];
+        if (img_term) {
+            float im = image[ib + e];
+            for (int g = 0; g < sb; ++g) im = im + cur[fb + g * L.vol + e] * rold[fb + g * L.vol + e];
+            image[ib + e] = im;
+        }
     }
 }
 
@@ -559,13 +567,14 @@ hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st) {
 }
 
 hipError_t launch_flush_imaging(const float* cur, const float* old, const float* rold, float* image, float* illum,
-                                const Layout& L, int nshots, long istride, int ill_terms, int img_term,
+                                const Layout& L, int nshots, int sb, long istride, int ill_terms, int img_term,
                                 hipStream_t st) {
-    if (ill_terms < 0 || ill_terms > 2 || nshots <= 0 || L.nz <= 8 || L.ny <= 8) return hipErrorInvalidValue;
+    if (ill_terms < 0 || ill_terms > 2 || nshots <= 0 || sb <= 0 || nshots % sb != 0 || L.nz <= 8 || L.ny <= 8)
+        return hipErrorInvalidValue;
     if (ill_terms == 0 && !img_term) return hipSuccess;
     const int n = (L.ny - 8) * L.px;
-    const dim3 grid((unsigned)std::min((n + 255) / 256, 64), (unsigned)(L.nz - 8), (unsigned)nshots);
-    flush_imaging_kernel<<<grid, 256, 0, st>>>(cur, old, rold, image, illum, L, istride, ill_terms, img_term);
+    const dim3 grid((unsigned)std::min((n + 255) / 256, 64), (unsigned)(L.nz - 8), (unsigned)(nshots / sb));
+    flush_imaging_kernel<<<grid, 256, 0, st>>>(cur, old, rold, image, illum, L, sb, istride, ill_terms, img_term);
     return hipGetLastError();
 }
 

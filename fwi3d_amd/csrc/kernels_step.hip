@@ -30,13 +30,14 @@
 // previous step before any of this step's.  The sums are bit for bit those
 // of per-step updates; the image/illumination traffic halves (single shot,
 // RTM: 44 -> 36 B per cell and step on average).
-// One shot group per block and no corrections: the illumination term of
-// the step before the previous one is known as well (its new source field
-// is this launch's old value o), so the illumination is updated every third
-// launch and the image every other one: IMG = 3 adds the image terms of the
-// previous step and this one, IMG = 4 the illumination terms of the two
-// previous steps (o*o, c*c) and this one, IMG = 5 both (RTM: 36 -> 34.7 B
-// per cell and step on average).  Terms still pending when a pass or a
+// Without corrections the illumination term of the step before the
+// previous one is known as well (its new source field is this launch's old
+// value o), so the illumination is updated every third launch and the
+// image every other one: IMG = 3 adds the image terms of the previous step
+// and this one, IMG = 4 the illumination terms of the two previous steps
+// (o*o, c*c) and this one, IMG = 5 both (single shot, RTM: 36 -> 34.7 B per
+// cell and step on average); batched launches again add the steps in order,
+// each step's terms in shot order.  Terms still pending when a pass or a
 // segment ends are added by a separate elementwise launch.
 //
 // Work decomposition (gfx950, wave64):
@@ -196,8 +197,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
     static_assert(IMG == 1 || (BWD && (!CORR || SB == 1)), "deferred imaging: backward launches only");
-    static_assert(IMG <= 2 || (SB == 1 && !CORR && (IMG == 4 || MODE == BWD_RTM)),
-                  "image every other launch, illumination every third: one shot group, no corrections");
+    static_assert(IMG <= 2 || (!CORR && (IMG == 4 || MODE == BWD_RTM)),
+                  "image every other launch, illumination every third: no corrections");
     // image (or gradient) and illumination read-modify-writes of this launch
     constexpr bool RMW_IMG = BWD && (IMG == 1 || IMG == 2 || IMG == 3 || IMG == 5 || MODE == BWD_FWI);
     constexpr bool RMW_ILL = BWD && (IMG == 1 || IMG == 2 || IMG == 4 || IMG == 5);
@@ -210,8 +211,13 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     // batched backward: the image/illumination terms of groups 1..SB-1,
     // double-buffered by plane parity, summed by group 0
     // (two terms per group and step; IMG = 2 passes two steps' terms, and
-    // group 0 parks its own terms of this step in the last two slots)
-    constexpr int NX = (BWD && SB > 1) ? (IMG == 2 ? 4 * (SB - 1) + 2 : 2 * (SB - 1)) : 1;
+    // group 0 parks its own terms of this step in the last two slots;
+    // IMG >= 3: five terms per group, slots 5k .. 5k+4 = the illumination
+    // terms of steps i-2, i-1, i, the image terms of steps i-1, i, and
+    // group 0 parks its illumination terms of i-1, i and image term of i)
+    constexpr int NX = (BWD && SB > 1)
+                           ? (IMG >= 3 ? 5 * (SB - 1) + 3 : IMG == 2 ? 4 * (SB - 1) + 2 : 2 * (SB - 1))
+                           : 1;
     __shared__ float xch[2][NX][TY][64];
 
     const Layout& L = a.L;
@@ -422,10 +428,35 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                     }
 #pragma unroll
                     for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][4 * k + 3][row][tx];
-                } else if constexpr (IMG >= 3) {
+                } else if constexpr (IMG >= 3 && SB == 1) {
                     // one group: this step's deferred terms, after the earlier ones
                     if constexpr (RMW_ILL) ill = ill + ill_c[r];
                     if constexpr (MODE == BWD_RTM && RMW_IMG) img = img + img_c[r];
+                } else if constexpr (IMG >= 3) {
+                    // step by step (i-2, i-1, i), each in shot order; group
+                    // 0's term of step i-2 (i-1 for the image) is in ill_r
+                    if constexpr (RMW_ILL) {
+#pragma unroll
+                        for (int k = 0; k < SB - 1; ++k) ill = ill + xch[xb][5 * k][row][tx];
+                        ill = ill + xch[xb][NX - 3][row][tx];
+#pragma unroll
+                        for (int k = 0; k < SB - 1; ++k) ill = ill + xch[xb][5 * k + 1][row][tx];
+                        ill = ill + xch[xb][NX - 2][row][tx];
+#pragma unroll
+                        for (int k = 0; k < SB - 1; ++k) ill = ill + xch[xb][5 * k + 2][row][tx];
+                    }
+                    if constexpr (MODE == BWD_RTM) {
+                        if constexpr (RMW_IMG) {
+#pragma unroll
+                            for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][5 * k + 3][row][tx];
+                            img = img + xch[xb][NX - 1][row][tx];
+#pragma unroll
+                            for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][5 * k + 4][row][tx];
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < SB - 1; ++k) img = img + xch[xb][5 * k + 4][row][tx];
+                    }
                 } else if constexpr (SB > 1) {
 #pragma unroll
                     for (int k = 0; k < SB - 1; ++k) {
@@ -574,22 +605,49 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                                 xch[buf][2 * (g - 1) + 1][row][tx] = t_img;
                         }
                     } else if constexpr (IMG >= 3) {
-                        // one group, no corrections: the illumination terms
-                        // of the two previous steps (new source fields o and
-                        // c), the image term of the previous one (c with the
-                        // receiver field it read, ro); this step's are added
-                        // after them in finish_plane
-                        if constexpr (RMW_ILL) {
-                            ill_r[r] = (il[r] + o[r] * o[r]) + c * c;
-                            ill_c[r] = t_ill;
-                        }
-                        if constexpr (MODE == BWD_RTM) {
-                            if constexpr (RMW_IMG) {
-                                img_r[r] = im[r] + c * ro[r];
-                                img_c[r] = t_img;
+                        // no corrections: the illumination terms of the two
+                        // previous steps (new source fields o and c), the
+                        // image term of the previous one (c with the
+                        // receiver field it read, ro); the later terms are
+                        // added after them in finish_plane
+                        const float p2 = o[r] * o[r], p1 = c * c;
+                        if (owner) {
+                            if constexpr (RMW_ILL) {
+                                if constexpr (SB == 1) {
+                                    ill_r[r] = (il[r] + p2) + p1;
+                                    ill_c[r] = t_ill;
+                                } else {
+                                    ill_r[r] = il[r] + p2;
+                                    xch[buf][NX - 3][row][tx] = p1;
+                                    xch[buf][NX - 2][row][tx] = t_ill;
+                                }
                             }
-                        } else {
-                            img_r[r] = im[r] + t_img;
+                            if constexpr (MODE == BWD_RTM) {
+                                if constexpr (RMW_IMG) {
+                                    img_r[r] = im[r] + c * ro[r];
+                                    if constexpr (SB == 1)
+                                        img_c[r] = t_img;
+                                    else
+                                        xch[buf][NX - 1][row][tx] = t_img;
+                                }
+                            } else {
+                                img_r[r] = im[r] + t_img;
+                            }
+                        } else if constexpr (SB > 1) {
+                            const int k = 5 * (g - 1);
+                            if constexpr (RMW_ILL) {
+                                xch[buf][k][row][tx] = p2;
+                                xch[buf][k + 1][row][tx] = p1;
+                                xch[buf][k + 2][row][tx] = t_ill;
+                            }
+                            if constexpr (MODE == BWD_RTM) {
+                                if constexpr (RMW_IMG) {
+                                    xch[buf][k + 3][row][tx] = c * ro[r];
+                                    xch[buf][k + 4][row][tx] = t_img;
+                                }
+                            } else {
+                                xch[buf][k + 4][row][tx] = t_img;
+                            }
                         }
                     } else {
                         // the previous step's terms: its new source field c
@@ -693,8 +751,8 @@ struct Entry {
     LaunchFn fn_img0;  // deferred imaging: none this launch
     LaunchFn fn_img2;  // deferred imaging: previous step's and this step's terms
     LaunchFn fn_corr_img0, fn_corr_img2;  // the same with the corrections (one shot group per block)
-    // one shot group, no corrections: image every other launch, illumination
-    // every third (IMG 3: image, 4: illumination, 5: both; 4 also for FWI)
+    // no corrections: image every other launch, illumination every third
+    // (IMG 3: image, 4: illumination, 5: both; 4 also for FWI)
     LaunchFn fn_img3, fn_img4, fn_img5;
 };
 
@@ -709,8 +767,15 @@ constexpr bool defers(int m, int by, int sb) {
     return m == BWD_RTM || (m == BWD_FWI && (sb == 1 || by * sb <= 8));
 }
 constexpr bool defers_corr(int m, int sb) { return is_bwd(m) && sb == 1; }
-constexpr bool cad3_ill(int m, int sb) { return is_bwd(m) && sb == 1; }
-constexpr bool cad3_img(int m, int sb) { return m == BWD_RTM && sb == 1; }
+// image every other launch, illumination every third: wherever deferred
+// imaging is, except in batched blocks of 1024 threads with two or more
+// rows per thread, whose instances would spill registers (those keep the
+// two-launch pattern)
+constexpr bool cad3_fits(int by, int rpt, int sb) { return sb == 1 || !(by * sb == 16 && rpt >= 2); }
+constexpr bool cad3_ill(int m, int by, int rpt, int sb) {
+    return is_bwd(m) && defers(m, by, sb) && cad3_fits(by, rpt, sb);
+}
+constexpr bool cad3_img(int m, int by, int rpt, int sb) { return m == BWD_RTM && cad3_fits(by, rpt, sb); }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
     {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
@@ -720,9 +785,12 @@ constexpr bool cad3_img(int m, int sb) { return m == BWD_RTM && sb == 1; }
                         : nullptr,                                                                     \
      defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 2 : 1> \
                         : nullptr,                                                                     \
-     cad3_img(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, SB) ? 3 : 1> : nullptr,         \
-     cad3_ill(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_ill(M, SB) ? 4 : 1> : nullptr,         \
-     cad3_img(M, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, SB) ? 5 : 1> : nullptr}
+     cad3_img(M, BY, RPT, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, BY, RPT, SB) ? 3 : 1>     \
+                              : nullptr,                                                              \
+     cad3_ill(M, BY, RPT, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_ill(M, BY, RPT, SB) ? 4 : 1>     \
+                              : nullptr,                                                              \
+     cad3_img(M, BY, RPT, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, BY, RPT, SB) ? 5 : 1>     \
+                              : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)

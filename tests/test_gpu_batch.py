@@ -169,12 +169,12 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
 @pytest.mark.parametrize("fwi", [False, True])
 @pytest.mark.parametrize("sb", [2, 4])
 def test_batched_deferred_imaging_identical(fl, sb, fwi):
-    """Batched RTM with image and illumination updated every other launch
-    (the default; the other groups hand group 0 two steps' terms), and
-    batched FWI with the illumination deferred (blocks of up to 512 threads;
-    larger ones update every step), equal
+    """Batched RTM with the image updated every other launch and the
+    illumination every third (the default; the other groups hand group 0
+    each step's terms), and batched FWI with the illumination deferred
+    (blocks of up to 512 threads; larger ones update every step), equal
     per-step updates bit for bit, over a whole pass and over the same
-    segments of odd and even length (each segment ends by folding the shot
+    segments of every length modulo 6 (each segment ends by folding the shot
     groups' partial sums into group 0, so segmented and whole passes round
     differently when there are several groups).  An odd number of planes
     and two z-chunks (of 18 and 17 planes) exercise the last-plane path of
@@ -183,7 +183,7 @@ def test_batched_deferred_imaging_identical(fl, sb, fwi):
     S = 4
     shots = _shots(case, S)
     outs = []
-    cuts_odd = [0, 5, 6, 17, 30, case.steps]
+    cuts_odd = [0, 5, 6, 8, 12, 15, 21, 30, case.steps]
     for defer, cuts in ((False, None), (True, None), (False, cuts_odd), (True, cuts_odd)):
         c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
         c.set_model_interior(case.interior)
