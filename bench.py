@@ -35,10 +35,13 @@ def log(msg):
 def measured_traffic(kernel_key):
     """HBM bytes per launch of one step-kernel shape, as measured by the two
     rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, scaled by the copy
-    kernel's calibration) summarised in profiles/*_hbm_traffic.txt; a
-    backward shape with deferred imaging alternates two instances (_IMG=0,
-    _IMG=2), whose mean is the per-launch figure.  None when no summary
-    covers the shape."""
+    kernel's calibration) summarised in profiles/*_hbm_traffic.txt.  A
+    backward shape with deferred imaging cycles through several instances:
+    every six RTM launches two without any update (_IMG=0), two with the
+    image only (_IMG=3), one with the illumination only (_IMG=4) and one
+    with both (_IMG=5); shapes that keep the two-launch pattern alternate
+    _IMG=0 and _IMG=2.  The weighted mean is the per-launch figure.  None
+    when no summary covers the shape."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_hbm_traffic.txt"))):
         found = {}
@@ -46,9 +49,10 @@ def measured_traffic(kernel_key):
             parts = line.split()
             if len(parts) == 3 and parts[0] == "traffic":
                 found[parts[1]] = int(parts[2])
-        pair = [found.get(kernel_key + "_IMG=%d" % i) for i in (0, 2)]
-        if all(pair):
-            return (pair[0] + pair[1]) // 2, os.path.relpath(path, ROOT)
+        for cycle in (((0, 2), (3, 2), (4, 1), (5, 1)), ((0, 1), (2, 1))):
+            got = [(found.get(kernel_key + "_IMG=%d" % i), w) for i, w in cycle]
+            if all(t for t, _ in got):
+                return sum(t * w for t, w in got) // sum(w for _, w in got), os.path.relpath(path, ROOT)
         if kernel_key in found:
             return found[kernel_key], os.path.relpath(path, ROOT)
     return None, None
