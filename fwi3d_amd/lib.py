@@ -205,8 +205,9 @@ class Context:
         self._chk(self.L.fwi_set_physics(self.h, flags))
 
     def set_deferred_imaging(self, enable):
-        """Image/illumination updated every other backward step (default on;
-        bit-identical results) or every step."""
+        """Image updated every other backward step and illumination every
+        third (every other step with the corrections; default on,
+        bit-identical results) or both every step."""
         self._chk(self.L.fwi_set_deferred_imaging(self.h, 1 if enable else 0))
 
     def set_model(self, vp_pad):
