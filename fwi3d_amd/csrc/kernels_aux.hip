@@ -306,32 +306,33 @@ __global__ void __launch_bounds__(256) flush_imaging_kernel(const float* __restr
                                                             const float* __restrict__ rold, float* __restrict__ image,
                                                             float* __restrict__ illum, Layout L, int sb, long istride,
                                                             int ill_terms, int img_term) {
-    const int z = 4 + blockIdx.y;
-    const long fb = (long)blockIdx.z * sb * L.vol + (long)z * L.pp;
-    const long ib = (long)blockIdx.z * istride + (long)z * L.pp;
     const int n = (L.ny - 8) * L.px;
-    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
-        const int y = 4 + q / L.px;
-        const int x = q - (y - 4) * L.px;
-        if (x < 4 || x >= L.nx - 4) continue;
-        const long e = (long)y * L.px + x;
-        if (ill_terms > 0) {
-            float il = illum[ib + e];
-            if (ill_terms > 1)
+    for (int z = 4 + blockIdx.y; z < L.nz - 4; z += gridDim.y) {
+        const long fb = (long)blockIdx.z * sb * L.vol + (long)z * L.pp;
+        const long ib = (long)blockIdx.z * istride + (long)z * L.pp;
+        for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+            const int y = 4 + q / L.px;
+            const int x = q - (y - 4) * L.px;
+            if (x < 4 || x >= L.nx - 4) continue;
+            const long e = (long)y * L.px + x;
+            if (ill_terms > 0) {
+                float il = illum[ib + e];
+                if (ill_terms > 1)
+                    for (int g = 0; g < sb; ++g) {
+                        const float o = old[fb + g * L.vol + e];
+                        il = il + o * o;
+                    }
                 for (int g = 0; g < sb; ++g) {
-                    const float o = old[fb + g * L.vol + e];
-                    il = il + o * o;
+                    const float c = cur[fb + g * L.vol + e];
+                    il = il + c * c;
                 }
-            for (int g = 0; g < sb; ++g) {
-                const float c = cur[fb + g * L.vol + e];
-                il = il + c * c;
+                illum[ib + e] = il;
             }
-            illum[ib + e] = il;
-        }
-        if (img_term) {
-            float im = image[ib + e];
-            for (int g = 0; g < sb; ++g) im = im + cur[fb + g * L.vol + e] * rold[fb + g * L.vol + e];
-            image[ib + e] = im;
+            if (img_term) {
+                float im = image[ib + e];
+                for (int g = 0; g < sb; ++g) im = im + cur[fb + g * L.vol + e] * rold[fb + g * L.vol + e];
+                image[ib + e] = im;
+            }
         }
     }
 }
@@ -573,7 +574,8 @@ hipError_t launch_flush_imaging(const float* cur, const float* old, const float*
         return hipErrorInvalidValue;
     if (ill_terms == 0 && !img_term) return hipSuccess;
     const int n = (L.ny - 8) * L.px;
-    const dim3 grid((unsigned)std::min((n + 255) / 256, 64), (unsigned)(L.nz - 8), (unsigned)(nshots / sb));
+    const dim3 grid((unsigned)std::min((n + 255) / 256, 64), (unsigned)std::min(L.nz - 8, 4096),
+                    (unsigned)(nshots / sb));
     flush_imaging_kernel<<<grid, 256, 0, st>>>(cur, old, rold, image, illum, L, sb, istride, ill_terms, img_term);
     return hipGetLastError();
 }
