@@ -329,12 +329,12 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     const int nsteps = nt + c->g.tlag;  // steps of the forward pass being reversed
     // deferred imaging: pairs of launches share one image/illumination
     // read-modify-write, and a pass (or segment) of odd length ends with a
-    // per-step launch; without corrections the image is updated every other
-    // launch and the illumination every third, and the terms still pending
-    // at the end are added by one elementwise launch.  Either way no term is
-    // left over when the pass ends.
+    // per-step launch; without aligned imaging the image is updated every
+    // other launch and the illumination every third, and the terms still
+    // pending at the end are added by one elementwise launch.  Either way no
+    // term is left over when the pass ends.
     const bool defer = c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
-    const bool defer3 = defer && fwi::step_defer3_supported(mode, c->arith, t, c->phys != 0);
+    const bool defer3 = defer && fwi::step_defer3_supported(mode, c->arith, t, c->phys);
     for (int b = b0; b < b1; ++b) {
         const int i = b - b0;
         if (defer3) {

@@ -89,7 +89,7 @@ struct StepArgs {
     int phys;              // backward: Physics bits
     // backward imaging of this launch: 1 = this step's terms; 0 = deferred
     // to a later launch; 2 = the previous step's terms, then this step's;
-    // without corrections: 3 = image of the previous step and this one,
+    // not with aligned imaging: 3 = image of the previous step and this one,
     // 4 = illumination of the two previous steps and this one, 5 = both
     // (other terms deferred)
     int img;
@@ -110,8 +110,9 @@ bool step_config_supported(int mode, int arith, const Tuning& t);
 // with or without the corrections of StepArgs::phys
 bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections);
 // the image every other launch and the illumination every third (StepArgs::img
-// 0, 3, 4, 5) available for this backward shape (no corrections)
-bool step_defer3_supported(int mode, int arith, const Tuning& t, bool corrections);
+// 0, 3, 4, 5) available for this backward shape and these Physics bits (not
+// with aligned imaging)
+bool step_defer3_supported(int mode, int arith, const Tuning& t, int phys);
 void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a, dim3& grid, dim3& block);
 
 // Host-side launchers (kernels_aux.hip).  "host layout" buffers here are

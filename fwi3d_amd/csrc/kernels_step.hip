@@ -30,7 +30,7 @@
 // previous step before any of this step's.  The sums are bit for bit those
 // of per-step updates; the image/illumination traffic halves (single shot,
 // RTM: 44 -> 36 B per cell and step on average).
-// Without corrections the illumination term of the step before the
+// Except with aligned imaging, the illumination term of the step before the
 // previous one is known as well (its new source field is this launch's old
 // value o), so the illumination is updated every third launch and the
 // image every other one: IMG = 3 adds the image terms of the previous step
@@ -197,8 +197,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
     static_assert(IMG == 1 || (BWD && (!CORR || SB == 1)), "deferred imaging: backward launches only");
-    static_assert(IMG <= 2 || (!CORR && (IMG == 4 || MODE == BWD_RTM)),
-                  "image every other launch, illumination every third: no corrections");
+    static_assert(IMG <= 2 || ((!CORR || SB == 1) && (IMG == 4 || MODE == BWD_RTM)),
+                  "image every other launch, illumination every third: backward launches only");
     // image (or gradient) and illumination read-modify-writes of this launch
     constexpr bool RMW_IMG = BWD && (IMG == 1 || IMG == 2 || IMG == 3 || IMG == 5 || MODE == BWD_FWI);
     constexpr bool RMW_ILL = BWD && (IMG == 1 || IMG == 2 || IMG == 4 || IMG == 5);
@@ -605,11 +605,13 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                                 xch[buf][2 * (g - 1) + 1][row][tx] = t_img;
                         }
                     } else if constexpr (IMG >= 3) {
-                        // no corrections: the illumination terms of the two
-                        // previous steps (new source fields o and c), the
-                        // image term of the previous one (c with the
-                        // receiver field it read, ro); the later terms are
-                        // added after them in finish_plane
+                        // the illumination terms of the two previous steps
+                        // (new source fields o and c), the image term of the
+                        // previous one (c with the receiver field it read,
+                        // ro); the later terms are added after them in
+                        // finish_plane.  Not with aligned imaging, whose
+                        // term of two steps back is no longer held (the host
+                        // launches these instances only without it).
                         const float p2 = o[r] * o[r], p1 = c * c;
                         if (owner) {
                             if constexpr (RMW_ILL) {
@@ -751,9 +753,10 @@ struct Entry {
     LaunchFn fn_img0;  // deferred imaging: none this launch
     LaunchFn fn_img2;  // deferred imaging: previous step's and this step's terms
     LaunchFn fn_corr_img0, fn_corr_img2;  // the same with the corrections (one shot group per block)
-    // no corrections: image every other launch, illumination every third
-    // (IMG 3: image, 4: illumination, 5: both; 4 also for FWI)
+    // image every other launch, illumination every third (IMG 3: image,
+    // 4: illumination, 5: both; 4 also for FWI)
     LaunchFn fn_img3, fn_img4, fn_img5;
+    LaunchFn fn_corr_img3, fn_corr_img4, fn_corr_img5;  // with the corrections (not aligned imaging)
 };
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }
@@ -776,6 +779,7 @@ constexpr bool cad3_ill(int m, int by, int rpt, int sb) {
     return is_bwd(m) && defers(m, by, sb) && cad3_fits(by, rpt, sb);
 }
 constexpr bool cad3_img(int m, int by, int rpt, int sb) { return m == BWD_RTM && cad3_fits(by, rpt, sb); }
+constexpr bool cad3_img_corr(int m, int sb) { return m == BWD_RTM && defers_corr(m, sb); }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
     {M, A, BY, RPT, SB, &launch_one<M, A, BY, RPT, SB, false, 1>,                                      \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
@@ -790,7 +794,13 @@ constexpr bool cad3_img(int m, int by, int rpt, int sb) { return m == BWD_RTM &&
      cad3_ill(M, BY, RPT, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_ill(M, BY, RPT, SB) ? 4 : 1>     \
                               : nullptr,                                                              \
      cad3_img(M, BY, RPT, SB) ? &launch_one<M, A, BY, RPT, SB, false, cad3_img(M, BY, RPT, SB) ? 5 : 1>     \
-                              : nullptr}
+                              : nullptr,                                                              \
+     cad3_img_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, cad3_img_corr(M, SB), cad3_img_corr(M, SB) ? 3 : 1> \
+                          : nullptr,                                                                  \
+     defers_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, defers_corr(M, SB), defers_corr(M, SB) ? 4 : 1>     \
+                        : nullptr,                                                                    \
+     cad3_img_corr(M, SB) ? &launch_one<M, A, BY, RPT, SB, cad3_img_corr(M, SB), cad3_img_corr(M, SB) ? 5 : 1> \
+                          : nullptr}
 #define FWI_TILES(M, A, SB)                                                                              \
     FWI_E(M, A, 4, 1, SB), FWI_E(M, A, 4, 2, SB), FWI_E(M, A, 8, 1, SB), FWI_E(M, A, 8, 2, SB),         \
         FWI_E(M, A, 4, 4, SB), FWI_E(M, A, 16, 1, SB), FWI_E(M, A, 16, 2, SB)
@@ -843,10 +853,12 @@ bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections
     return corrections ? (e->fn_corr_img0 && e->fn_corr_img2) : (e->fn_img0 && e->fn_img2);
 }
 
-bool step_defer3_supported(int mode, int arith, const Tuning& t, bool corrections) {
+bool step_defer3_supported(int mode, int arith, const Tuning& t, int phys) {
     const Entry* e = find(mode, arith, t);
-    if (!e || corrections || !e->fn_img0 || !e->fn_img4) return false;
-    return mode == BWD_FWI || (e->fn_img3 && e->fn_img5);
+    if (!e || (phys & PHYS_ALIGNED_IMAGING)) return false;
+    if (phys)
+        return e->fn_corr_img0 && e->fn_corr_img4 && (mode == BWD_FWI || (e->fn_corr_img3 && e->fn_corr_img5));
+    return e->fn_img0 && e->fn_img4 && (mode == BWD_FWI || (e->fn_img3 && e->fn_img5));
 }
 
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st) {
@@ -854,15 +866,16 @@ hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, 
     if (!e || a.nshots % t.sb != 0) return hipErrorInvalidConfiguration;
     if (a.img != 1 && !is_bwd(mode)) return hipErrorInvalidConfiguration;
     if (a.img < 0 || a.img > 5) return hipErrorInvalidConfiguration;
+    if (a.img >= 3 && (a.phys & PHYS_ALIGNED_IMAGING)) return hipErrorInvalidConfiguration;
     const bool corr = is_bwd(mode) && a.phys != 0;
     LaunchFn fn = nullptr;
     switch (a.img) {
         case 0: fn = corr ? e->fn_corr_img0 : e->fn_img0; break;
         case 1: fn = corr ? e->fn_corr : e->fn; break;
         case 2: fn = corr ? e->fn_corr_img2 : e->fn_img2; break;
-        case 3: fn = corr ? nullptr : e->fn_img3; break;
-        case 4: fn = corr ? nullptr : e->fn_img4; break;
-        default: fn = corr ? nullptr : e->fn_img5; break;
+        case 3: fn = corr ? e->fn_corr_img3 : e->fn_img3; break;
+        case 4: fn = corr ? e->fn_corr_img4 : e->fn_img4; break;
+        default: fn = corr ? e->fn_corr_img5 : e->fn_img5; break;
     }
     if (!fn) return hipErrorInvalidConfiguration;
     StepArgs args = a;

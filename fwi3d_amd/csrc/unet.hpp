@@ -18,9 +18,12 @@ hipError_t conv_fwd_f32out(const bf16* inA, int Ca, const bf16* inB, int Cb, con
 // din[Cin] += W^T * dout[Cout]; wd: BF16 [roundup16(Cin)][Kpadd], k = tap'*Cout + co (mirrored taps)
 hipError_t conv_dgrad(const float* dout, int Cout, const bf16* wd, int Kpadd, int Cin, float* din, int B, int D,
                       int H, int W, hipStream_t st);
-// dw[co][tap*Cin + ci] += sum_v dout[v][co] * concat(inA,inB)[v+tap][ci];  dw row stride = ldw
+// dw[co][tap*Cin + ci] += sum_v dout[v][co] * concat(inA,inB)[v+tap][ci];  dw row stride = ldw.
+// Deterministic: partial sums of at most kWgradChunks row chunks go to part
+// (part_cap floats, at least chunks * Cout * ldw) and are added in order.
+constexpr int kWgradChunks = 64;
 hipError_t conv_wgrad(const float* dout, int Cout, const bf16* inA, int Ca, const bf16* inB, int Cb, float* dw,
-                      int ldw, int B, int D, int H, int W, hipStream_t st);
+                      int ldw, int B, int D, int H, int W, float* part, long part_cap, hipStream_t st);
 hipError_t relu_bwd(const float* dact, const bf16* act, float* dpre, long n, hipStream_t st);
 hipError_t bias_grad(const float* d, long nvox, int C, float* db, hipStream_t st);
 hipError_t avgpool_fwd(const bf16* in, bf16* out, int B, int D, int H, int W, int C, hipStream_t st);
@@ -28,8 +31,9 @@ hipError_t avgpool_bwd(const float* dout, float* din, int B, int D, int H, int W
 hipError_t upsample_fwd(const bf16* in, bf16* out, int B, int D, int H, int W, int C, hipStream_t st);
 hipError_t upsample_bwd(const float* dout, float* din, int B, int D, int H, int W, int C, hipStream_t st);
 hipError_t head_fwd(const bf16* act, int C, const float* w, float bias, float* y, long nvox, hipStream_t st);
+// deterministic like conv_wgrad: part holds at least 256 * 17 floats
 hipError_t head_bwd(const float* dy, const bf16* act, int C, const float* w, float* dact, float* dw, float* db,
-                    long nvox, hipStream_t st);
+                    long nvox, float* part, long part_cap, hipStream_t st);
 hipError_t mse(const float* y, const float* t, float* dy, long n, double* partial, double* out, hipStream_t st);
 hipError_t adam(float* p, const float* g, float* m1, float* m2, long n, float lr, float b1, float b2, float eps,
                 int step, hipStream_t st);

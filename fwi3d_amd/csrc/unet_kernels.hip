@@ -20,6 +20,7 @@
 //               (M = 16 co, N = 16 ci, K = 32 voxels along x per MFMA)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "unet.hpp"
@@ -120,7 +121,7 @@ template <typename TI>
 __global__ void __launch_bounds__(256) conv_wgrad_kernel(const float* __restrict__ dout, int Cout,
                                                          const TI* __restrict__ inA, int Ca,
                                                          const bf16* __restrict__ inB, int Cb,
-                                                         float* __restrict__ dw, int Kpad, int B, int D, int H,
+                                                         float* __restrict__ part, int Kpad, int B, int D, int H,
                                                          int W, int rows_per_block) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -162,14 +163,35 @@ __global__ void __launch_bounds__(256) conv_wgrad_kernel(const float* __restrict
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
         }
     }
-    // D[m = co][n = ci]: lane holds rows g*4 + i, column m
+    // D[m = co][n = ci]: lane holds rows g*4 + i, column m.  The four
+    // waves' tiles, summed in wave order, are this block's partial sum for
+    // its row chunk (blockIdx.y); wgrad_reduce_kernel adds the chunks in
+    // order, so the result does not depend on scheduling.
+    __shared__ float red[4][4][64];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][i][lane] = acc[i];
+    __syncthreads();
     const int cw = cib * 16 + m;
-    if (cw < Cin) {
+    if (wave == 0 && cw < Cin) {
+        float* pb = part + (long)blockIdx.y * Cout * Kpad;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int cr = cob * 16 + g * 4 + i;
-            atomicAdd(&dw[(long)cr * Kpad + tap * Cin + cw], acc[i]);
+            pb[(long)cr * Kpad + tap * Cin + cw] = ((red[0][i][lane] + red[1][i][lane]) + red[2][i][lane]) +
+                                                   red[3][i][lane];
         }
+    }
+}
+
+// dw[e] += sum over row chunks k (in order) of part[k][e], for the columns
+// e % Kpad < kvalid that the chunks wrote
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ part, int nchunks, long n,
+                                                           int Kpad, int kvalid, float* __restrict__ dw) {
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) {
+        if ((int)(e % Kpad) >= kvalid) continue;
+        float s = 0.0f;
+        for (int k = 0; k < nchunks; ++k) s += part[(long)k * n + e];
+        dw[e] += s;
     }
 }
 
@@ -193,7 +215,7 @@ __global__ void bias_grad_kernel(const float* __restrict__ d, long nvox, int C, 
         if (threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&db[c], sh[0]);
+    if (threadIdx.x == 0) db[c] += sh[0];  // one block per channel: no other writer
 }
 
 // 2x2x2 average pooling (out dims D/2 x H/2 x W/2)
@@ -279,9 +301,11 @@ __global__ void head_fwd_kernel(const bf16* __restrict__ act, int C, const float
 }
 
 // dAct[v][c] = dy[v]*w[c];  dw[c] += sum_v dy[v]*act[v][c];  db += sum_v dy[v]
+// (per-block partial sums of dw and db into bpart[block][0..C]; head_reduce_kernel
+// adds the blocks in order)
 __global__ void head_bwd_kernel(const float* __restrict__ dy, const bf16* __restrict__ act, int C,
-                                const float* __restrict__ w, float* __restrict__ dact, float* __restrict__ dw,
-                                float* __restrict__ db, long nvox) {
+                                const float* __restrict__ w, float* __restrict__ dact, float* __restrict__ bpart,
+                                long nvox) {
     __shared__ float sh[17][256];
     float part[17];
     for (int c = 0; c <= C; ++c) part[c] = 0.0f;
@@ -300,10 +324,20 @@ __global__ void head_bwd_kernel(const float* __restrict__ dy, const bf16* __rest
             for (int c = 0; c <= C; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < C; ++c) atomicAdd(&dw[c], sh[c][0]);
-        atomicAdd(db, sh[C][0]);
-    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c <= C; ++c) bpart[blockIdx.x * 17 + c] = sh[c][0];
+}
+
+__global__ void head_reduce_kernel(const float* __restrict__ part, int nblocks, int C, float* __restrict__ dw,
+                                   float* __restrict__ db) {
+    const int c = threadIdx.x;
+    if (c > C) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblocks; ++b) s += part[b * 17 + c];
+    if (c < C)
+        dw[c] += s;
+    else
+        *db += s;
 }
 
 // per-block partial sums of (y - t)^2 in FP64 and dy = 2 (y - t) / n
@@ -466,13 +500,21 @@ hipError_t conv_dgrad(const float* dout, int Cout, const bf16* wd, int Kpadd, in
 }
 
 hipError_t conv_wgrad(const float* dout, int Cout, const bf16* inA, int Ca, const bf16* inB, int Cb, float* dw,
-                      int ldw, int B, int D, int H, int W, hipStream_t st) {
+                      int ldw, int B, int D, int H, int W, float* part, long part_cap, hipStream_t st) {
     if (Cout % 16 != 0) return hipErrorInvalidValue;
     const int Cin = Ca + Cb;
+    if (ldw < 27 * Cin) return hipErrorInvalidValue;
     const int rows = B * D * H;
-    const int rpb = 16;
-    dim3 grid(27 * ((Cin + 15) / 16) * (Cout / 16), (rows + rpb - 1) / rpb);
-    conv_wgrad_kernel<bf16><<<grid, 256, 0, st>>>(dout, Cout, inA, Ca, inB, Cb, dw, ldw, B, D, H, W, rpb);
+    // row chunks of at least 16 rows, at most kWgradChunks of them
+    const int nchunks = std::min(kWgradChunks, (rows + 15) / 16);
+    const int rpb = (rows + nchunks - 1) / nchunks;
+    const int nck = (rows + rpb - 1) / rpb;
+    const long n = (long)Cout * ldw;
+    if ((long)nck * n > part_cap) return hipErrorInvalidValue;
+    dim3 grid(27 * ((Cin + 15) / 16) * (Cout / 16), nck);
+    conv_wgrad_kernel<bf16><<<grid, 256, 0, st>>>(dout, Cout, inA, Ca, inB, Cb, part, ldw, B, D, H, W, rpb);
+    wgrad_reduce_kernel<<<(unsigned)std::min<long>((n + 255) / 256, 1024), 256, 0, st>>>(part, nck, n, ldw, 27 * Cin,
+                                                                                         dw);
     return hipGetLastError();
 }
 
@@ -482,7 +524,7 @@ hipError_t relu_bwd(const float* dact, const bf16* act, float* dpre, long n, hip
 }
 
 hipError_t bias_grad(const float* d, long nvox, int C, float* db, hipStream_t st) {
-    bias_grad_kernel<<<dim3(256, C), 256, 0, st>>>(d, nvox, C, db);
+    bias_grad_kernel<<<dim3(1, C), 256, 0, st>>>(d, nvox, C, db);
     return hipGetLastError();
 }
 
@@ -512,9 +554,10 @@ hipError_t head_fwd(const bf16* act, int C, const float* w, float bias, float* y
 }
 
 hipError_t head_bwd(const float* dy, const bf16* act, int C, const float* w, float* dact, float* dw, float* db,
-                    long nvox, hipStream_t st) {
-    if (C > 16) return hipErrorInvalidValue;
-    head_bwd_kernel<<<256, 256, 0, st>>>(dy, act, C, w, dact, dw, db, nvox);
+                    long nvox, float* part, long part_cap, hipStream_t st) {
+    if (C > 16 || part_cap < 256 * 17) return hipErrorInvalidValue;
+    head_bwd_kernel<<<256, 256, 0, st>>>(dy, act, C, w, dact, part, nvox);
+    head_reduce_kernel<<<1, 64, 0, st>>>(part, 256, C, dw, db);
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 // then b[Cout]; then the head w[16], b.  Taps are ordered
 // tap = (dz+1)*9 + (dy+1)*3 + (dx+1).
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <random>
 #include <string>
@@ -53,6 +54,9 @@ struct fwi_unet {
     int adam_step = 0;
     double* partial = nullptr;
     double* result = nullptr;
+    // per-chunk partial sums of the weight gradients (deterministic backward)
+    float* wpart = nullptr;
+    long wpart_cap = 0;
     // activations (BF16) and gradients (FP32) for the current shape
     int B = 0, D = 0, H = 0, W = 0;
     bool grads = false;
@@ -223,7 +227,8 @@ int backward(fwi_unet* n, double* loss) {
         const int cin = s.ca + s.cb;
         UCHK(n, relu_bwd(dact, act, dact, nvox * s.cout, st));
         UCHK(n, bias_grad(dact, nvox, s.cout, n->g + n->boff[l], st));
-        UCHK(n, conv_wgrad(dact, s.cout, ina, s.ca, inb, s.cb, n->g + n->woff[l], 27 * cin, B, d, h, w, st));
+        UCHK(n, conv_wgrad(dact, s.cout, ina, s.ca, inb, s.cb, n->g + n->woff[l], 27 * cin, B, d, h, w, n->wpart,
+                           n->wpart_cap, st));
         if (!din) return FWI_OK;
         UCHK(n, hipMemsetAsync(din, 0, nvox * cin * sizeof(float), st));
         UCHK(n, conv_dgrad(dact, s.cout, n->wd[l], n->kpadd[l], cin, din, B, d, h, w, st));
@@ -235,7 +240,7 @@ int backward(fwi_unet* n, double* loss) {
         return FWI_OK;
     };
     UCHK(n, head_bwd(n->dy, n->d1b, kHeadC, n->p + n->hoff, n->gd1b, n->g + n->hoff, n->g + n->hoff + kHeadC, v1,
-                     st));
+                     n->wpart, n->wpart_cap, st));
     URC(layer(9, n->gd1b, n->d1b, v1, n->d1a, nullptr, n->gd1a, D, H, W));
     URC(layer(8, n->gd1a, n->d1a, v1, n->u1, n->e1b, n->gcat1, D, H, W));
     URC(slice(n->gcat1, 48, 0, n->gu1, 32, v1));
@@ -315,8 +320,12 @@ int fwi_unet_create(int device, uint64_t seed, fwi_unet** out) {
     }
     std::normal_distribution<float> hd(0.0f, 0.05f);
     for (int c = 0; c < kHeadC; ++c) h[n->hoff + c] = hd(rng);
+    long wmax = 256 * 17;
+    for (int l = 0; l < 10; ++l)
+        wmax = std::max(wmax, (long)fwi::unet::kWgradChunks * kConvs[l].cout * 27 * (kConvs[l].ca + kConvs[l].cb));
+    n->wpart_cap = wmax;
     if (ualloc(n, &n->p, off) || ualloc(n, &n->g, off) || ualloc(n, &n->m1, off) || ualloc(n, &n->m2, off) ||
-        ualloc(n, &n->partial, 1024) || ualloc(n, &n->result, 4))
+        ualloc(n, &n->partial, 1024) || ualloc(n, &n->result, 4) || ualloc(n, &n->wpart, wmax))
         return bail(FWI_ERR_HIP);
     for (int l = 0; l < 10; ++l) {
         const ConvSpec& s = kConvs[l];
@@ -342,6 +351,7 @@ int fwi_unet_destroy(fwi_unet* n) {
     ufree(n->m2);
     ufree(n->partial);
     ufree(n->result);
+    ufree(n->wpart);
     for (int l = 0; l < 10; ++l) {
         ufree(n->wt[l]);
         ufree(n->wd[l]);

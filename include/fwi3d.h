@@ -94,15 +94,16 @@ int fwi_set_arith(fwi_ctx* ctx, int arith);
 #define FWI_PHYS_ALIGNED_IMAGING 4   /* image/illumination from the source field at the receiver field's time */
 int fwi_set_physics(fwi_ctx* ctx, int flags);
 
-/* Backward passes update image and illumination every other time step,
- * adding two steps' terms in step order (enable = 1, the default; results
- * are bit-identical to per-step updates) or every step (0).  With one shot
- * per block and no corrections the illumination is updated every third
- * step (three terms in step order), and terms still pending when a pass or
- * a segment of backward_steps ends are added by one extra launch.  Batched
- * passes with corrections, and batched FWI passes in blocks of more than
- * 512 threads, always update every step; the FWI gradient term is always
- * added per step, only its illumination deferred.  No reference counterpart: the
+/* Backward passes update the image every other time step and the
+ * illumination every third, adding the held terms in step order (enable =
+ * 1, the default; results are bit-identical to per-step updates), or both
+ * every step (0).  Terms still pending when a pass or a segment of
+ * backward_steps ends are added by one extra launch.  With aligned imaging,
+ * and in batched blocks of 1024 threads with two or more rows per thread,
+ * both are updated every other step; batched passes with corrections, and
+ * batched FWI passes in blocks of more than 512 threads, update every step;
+ * the FWI gradient term is always added per step, only its illumination
+ * deferred.  No reference counterpart: the
  * reference's cross_correlation (migration.cu:181-202) and build_gradient
  * (inversion.cu:491-558) update them in every step. */
 int fwi_set_deferred_imaging(fwi_ctx* ctx, int enable);

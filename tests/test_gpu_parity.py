@@ -89,15 +89,17 @@ def test_backward_bit_exact(fl, fwi):
     assert np.array_equal(g_Ro, Ro)
 
 
-@pytest.mark.parametrize("phys", [0, 7])
+@pytest.mark.parametrize("phys", [0, 3, 7])
 @pytest.mark.parametrize("arith", ["reference", "fp32"])
 @pytest.mark.parametrize("fwi", [False, True])
 def test_deferred_imaging_identical(fl, fwi, arith, phys):
     """Image/illumination (gradient/illumination) updated every second or
     third backward launch, the default, equal per-step updates bit for bit:
     over a whole pass and over segments of every length modulo 6 (a segment
-    never leaves a term behind), with and without the three corrections; with
-    reference arithmetic both equal the oracle.
+    never leaves a term behind), without corrections, with source
+    subtraction and the Dyy gradient term (same cadence), and with all three
+    (aligned imaging: every other launch); with reference arithmetic both
+    equal the oracle.
     An odd number of planes in two z-chunks (20 and 19) exercises the
     last-plane path of the parity-unrolled loop."""
     case = Case(nz=31, nt=61)

@@ -68,6 +68,25 @@ def test_gradients_match_oracle(net):
     assert rel_l2(g[hoff:], rg[hoff:]) <= 2e-2
 
 
+def test_gradients_and_training_are_deterministic():
+    """The weight, bias and head gradients are summed in a fixed order (no
+    float atomics), so the same inputs give the same bits, and two trainings
+    from the same seed end with identical parameters."""
+    from fwi3d_amd.unet import UNet
+    x3, t = _batch(3, B=4, D=16, H=32, W=32)  # the training patch shape: 64 row chunks
+    params = []
+    for _ in range(2):
+        with UNet(seed=9) as n:
+            l1, g1 = n.gradients(x3, t)
+            l2, g2 = n.gradients(x3, t)
+            assert l1 == l2 and np.array_equal(g1, g2)
+            for it in range(10):
+                xb, tb = _batch(200 + it % 4)
+                n.train_step(xb, tb, 2e-3)
+            params.append(n.params.copy())
+    assert np.array_equal(params[0], params[1])
+
+
 def test_training_lowers_loss():
     from fwi3d_amd.unet import UNet
     with UNet(seed=5) as n:
