@@ -280,7 +280,9 @@ long fwi_unet_num_params(const fwi_unet* net);
 int fwi_unet_get_params(fwi_unet* net, float* out);
 int fwi_unet_set_params(fwi_unet* net, const float* in);
 int fwi_unet_forward(fwi_unet* net, const float* x3, int B, int D, int H, int W, float* y);
-/* MSE loss mean((y - target)^2) and the gradient of every parameter. */
+/* MSE loss mean((y - target)^2) and the gradient of every parameter.
+ * Deterministic: the gradient sums run in a fixed order (no float atomics),
+ * so equal inputs and parameters give equal bits. */
 int fwi_unet_gradients(fwi_unet* net, const float* x3, const float* target, int B, int D, int H, int W,
                        double* loss, float* grad);
 /* One Adam step (beta 0.9 / 0.999) on the MSE loss. */
