@@ -1,6 +1,7 @@
 """Build script for the native parts of fwi3d_amd.
 
-    python -m fwi3d_amd.build            # HIP library + host library + drivers
+    python -m fwi3d_amd.build                  # HIP library + host library + drivers
+    python -m fwi3d_amd.build --incremental    # recompile only changed HIP sources
 
 Outputs (in-tree, git-ignored):
     fwi3d_amd/libfwi3d_hip.so     hand-written gfx950 kernels + C ABI (include/fwi3d.h)
@@ -48,16 +49,34 @@ def _run(cmd, cwd=None):
     return out
 
 
-def build_hip():
+def build_hip(incremental=False):
+    """Compile the HIP sources (in parallel) and link libfwi3d_hip.so.  With
+    incremental=True the objects are kept in fwi3d_amd/_build and a source is
+    recompiled only when it, or a header of csrc/ or include/, is newer than
+    its object."""
+    from concurrent.futures import ThreadPoolExecutor
+    odir = os.path.join(PKG, "_build") if incremental else CSRC
+    os.makedirs(odir, exist_ok=True)
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    headers += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    newest_header = max(os.path.getmtime(h) for h in headers)
+    jobs = []
     objs = []
     for src in HIP_SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(odir, src.replace(".hip", ".o"))
         objs.append(obj)
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(path),
+                                                                                newest_header):
+            continue
+        jobs.append([HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj])
+    with ThreadPoolExecutor(max_workers=len(jobs) or 1) as ex:
+        list(ex.map(_run, jobs))
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", HIP_LIB + ".tmp"])
     os.replace(HIP_LIB + ".tmp", HIP_LIB)
-    for o in objs:
-        os.remove(o)
+    if not incremental:
+        for o in objs:
+            os.remove(o)
     return HIP_LIB
 
 
@@ -77,11 +96,11 @@ def build_host():
     return HOST_LIB
 
 
-def build_all():
-    build_hip()
+def build_all(incremental=False):
+    build_hip(incremental)
     build_host()
 
 
 if __name__ == "__main__":
-    build_all()
+    build_all(incremental="--incremental" in sys.argv[1:])
     print("built", HIP_LIB)

@@ -248,24 +248,30 @@ def two_layer_101():
     return Workload("two_layer", interior, 50, dh, 1e-3, 1000, 45.0, np.array([[500.0, 500.0, 20.0]]), [rec])
 
 
-def gradient_model_401(seed=1):
-    nx, ny, nz = 401, 401, 201
-    dh = 10.0
+def gradient_model(nx, ny, nz, dh, zmax, depths, centre, sigma, seed=1):
+    """Linear gradient 1500 -> 3500 m/s over zmax metres of depth, 5 seeded
+    dipping planar layers of +-300 m/s at `depths` (dips up to 0.1 in x and y
+    about the model centre) and a Gaussian anomaly of +400 m/s (std `sigma`)
+    at `centre` (x, y, z in metres); (ny, nx, nz) float32."""
     rng = np.random.default_rng(seed)
     x = np.arange(nx) * dh
     y = np.arange(ny) * dh
     z = np.arange(nz) * dh
     Y, X, Z = np.meshgrid(y, x, z, indexing="ij")
-    v = 1500.0 + 2000.0 * Z / 2000.0
-    depths = np.linspace(300.0, 1700.0, 5)
+    v = 1500.0 + 2000.0 * Z / zmax
     xc, yc = x.mean(), y.mean()
     for l, z0 in enumerate(depths):
         ax, ay = rng.uniform(-0.1, 0.1, 2)
         surf = z0 + ax * (X - xc) + ay * (Y - yc)
         v += np.where(Z >= surf, 300.0 if l % 2 == 0 else -300.0, 0.0)
-    r2 = (X - 2000.0) ** 2 + (Y - 2000.0) ** 2 + (Z - 1200.0) ** 2
-    v += 400.0 * np.exp(-0.5 * r2 / 300.0 ** 2)
+    r2 = (X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2
+    v += 400.0 * np.exp(-0.5 * r2 / sigma ** 2)
     return np.ascontiguousarray(v.astype(np.float32))
+
+
+def gradient_model_401(seed=1):
+    return gradient_model(401, 401, 201, 10.0, 2000.0, np.linspace(300.0, 1700.0, 5), (2000.0, 2000.0, 1200.0),
+                          300.0, seed)
 
 
 def gradient_401():
@@ -290,6 +296,37 @@ def gradient_401_16(interior=None):
             X, Y = np.meshgrid(gx, gy, indexing="ij")
             recs.append(np.stack([X.ravel(), Y.ravel(), np.full(X.size, 20.0)], axis=1))
     return Workload("gradient16", interior, 50, 10.0, 0.8e-3, 3000, 60.0, np.array(shots), recs)
+
+
+def gradient_survey(nx=101, ny=101, nz=51, nb=20, nt=200, dh=10.0, patch=25, spacing=20.0, full_spread=False):
+    """A shrunken version of the 16-shot survey on the gradient / dipping-layer
+    / anomaly model: interior nx x ny x nz at dh, the model's depth features
+    scaled to the grid, 16 shots on a 4 x 4 surface grid at 20 m depth, each
+    with a (2 patch + 1)^2 receiver patch at `spacing` centred on the shot and
+    clipped at the model edges (full_spread: every shot records the whole
+    surface grid instead, as one RPS file of the reference drivers does);
+    dt 0.8 ms, fmax 60 Hz (20 Hz Ricker peak), nt samples."""
+    zmax = (nz - 1) * dh
+    xmax, ymax = (nx - 1) * dh, (ny - 1) * dh
+    interior = gradient_model(nx, ny, nz, dh, zmax, np.linspace(0.15 * zmax, 0.85 * zmax, 5),
+                              (xmax / 2, ymax / 2, 0.6 * zmax), 0.15 * min(xmax, zmax))
+    shots, recs = [], []
+    fx = [xmax * f for f in (0.125, 0.375, 0.625, 0.875)]
+    fy = [ymax * f for f in (0.125, 0.375, 0.625, 0.875)]
+    for sy in fy:
+        for sx in fx:
+            shots.append([sx, sy, 20.0])
+            if full_spread:
+                gx = np.arange(0.0, xmax + 1e-6, spacing)
+                gy = np.arange(0.0, ymax + 1e-6, spacing)
+            else:
+                gx = sx + np.arange(-patch, patch + 1) * spacing
+                gy = sy + np.arange(-patch, patch + 1) * spacing
+                gx = gx[(gx >= 0.0) & (gx <= xmax)]
+                gy = gy[(gy >= 0.0) & (gy <= ymax)]
+            X, Y = np.meshgrid(gx, gy, indexing="ij")
+            recs.append(np.stack([X.ravel(), Y.ravel(), np.full(X.size, 20.0)], axis=1))
+    return Workload("gradient16s", interior, nb, dh, 0.8e-3, nt, 60.0, np.array(shots), recs)
 
 
 def tiny(nshots=2, nx=40, ny=36, nz=30, nt=160):

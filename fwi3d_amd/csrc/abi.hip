@@ -432,8 +432,8 @@ int check_ready_forward(fwi_ctx* c, int boundary) {
     if (boundary != FWI_BOUNDARY_ABC && boundary != FWI_BOUNDARY_RBC) return fail(c, FWI_ERR_ARG, "boundary");
     if (!c->have_model) return fail(c, FWI_ERR_STATE, "model not set");
     if ((int)c->wavelet.size() != c->g.nt) return fail(c, FWI_ERR_STATE, "wavelet not set");
-    for (int s = 0; s < c->nshots; ++s)
-        if (c->sx[s] < 0) return fail(c, FWI_ERR_STATE, "source not set for every shot slot");
+    for (int s = 0; s < c->nshots; ++s)  // -2: a disabled slot (fwi_shot_disable)
+        if (c->sx[s] == -1) return fail(c, FWI_ERR_STATE, "source not set for every shot slot");
     if (boundary == FWI_BOUNDARY_ABC && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
     return sync_geometry(c);
 }
@@ -825,6 +825,20 @@ int fwi_shot_set_source(fwi_ctx* c, int slot, int sx, int sy, int sz) {
 }
 
 int fwi_set_source(fwi_ctx* c, int sx, int sy, int sz) { return fwi_shot_set_source(c, 0, sx, sy, sz); }
+
+int fwi_shot_disable(fwi_ctx* c, int slot) {
+    if (!c) return FWI_ERR_ARG;
+    CHKRC(check_slot(c, slot));
+    HIPCHK(c, hipSetDevice(c->device));
+    c->sx[slot] = c->sy[slot] = c->sz[slot] = -2;
+    const long off = -1;  // no source cell: the injection kernels skip the slot
+    HIPCHK(c, hipMemcpy(c->src_off + slot, &off, sizeof off, hipMemcpyHostToDevice));
+    c->rx[slot].clear();
+    c->ry[slot].clear();
+    c->rz[slot].clear();
+    c->geom_dirty = true;
+    return sync_geometry(c);
+}
 
 int fwi_shot_set_receivers(fwi_ctx* c, int slot, int nrec, const int* rx, const int* ry, const int* rz) {
     if (!c) return FWI_ERR_ARG;

@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) blob_boundary_kernel(float* __restrict__ 
 // at s*sstride and its source cell is off[s].
 __global__ void add_points_kernel(float* dev, const long* __restrict__ off, int n, long sstride, float value) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) {
+    if (s < n && off[s] >= 0) {  // a negative offset: the slot has no source
         float* p = dev + (long)s * sstride + off[s];
         *p = *p + value;
     }

@@ -2,6 +2,7 @@
 #include "inversion.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <iostream>
 #include <stdexcept>
@@ -40,6 +41,10 @@ void Inversion::set_parameters() {
     if (upload == "updated") upload_updated = true;
     else if (upload == "last_trial") upload_updated = false;
     else throw std::invalid_argument("fwi_model_upload must be 'updated' or 'last_trial', not '" + upload + "'");
+    const std::string exg = catch_parameter_or("inv_export_gradient", parameters, "false");
+    if (exg == "true") export_gradients = true;
+    else if (exg == "false") export_gradients = false;
+    else throw std::invalid_argument("inv_export_gradient must be 'true' or 'false', not '" + exg + "'");
 
     create_context(rbc_nb, &rbc);
     create_context(abc_nb, &ctx);  // leaves nb / nxx / nyy / nzz at the ABC grid
@@ -65,7 +70,8 @@ void Inversion::set_obs_data(int shot) {
 }
 
 // Modeling::initialization on context c (boundary cells nbc) for the shots
-// [first, first + count); unused slots repeat the first shot.
+// [first, first + count); the unused slots of a partial batch are disabled
+// (no source, no receivers), so they add nothing to gradient or illumination.
 void Inversion::place_shots(fwi_ctx* c, int nbc, int first, int count) {
     auto idx = [&](float v) { return (int)((v + 0.5f * dh) / dh) + nbc; };
     std::vector<int> rx(geometry->nrec), ry(geometry->nrec), rz(geometry->nrec);
@@ -75,7 +81,11 @@ void Inversion::place_shots(fwi_ctx* c, int nbc, int first, int count) {
         rz[r] = idx(geometry->zrec[r]);
     }
     for (int slot = 0; slot < batch; slot++) {
-        const int shot = first + (slot < count ? slot : 0);
+        if (slot >= count) {
+            check(fwi_shot_disable(c, slot), "fwi_shot_disable");
+            continue;
+        }
+        const int shot = first + slot;
         check(fwi_shot_set_source(c, slot, idx(geometry->xsrc[shot]), idx(geometry->ysrc[shot]),
                                   idx(geometry->zsrc[shot])),
               "fwi_shot_set_source");
@@ -114,14 +124,9 @@ void Inversion::compute_gradient() {
         stage_info = "Wavefield reconstruction: forward and backward propagation";
         show_inv_info();
         place_shots(rbc, rbc_nb, srcId, count);
-        for (int slot = 0; slot < batch; slot++) {  // update_RBC, :204-216
-            const int shot = srcId + (slot < count ? slot : 0);
-            random_boundary(rbc, slot, shot, rbc_ratio, rbc_varVp);
-            if (slot < count) {
-                check(fwi_shot_get_backward_data(ctx, slot, residual.data()), "fwi_shot_get_backward_data");
-            } else {
-                std::fill(residual.begin(), residual.end(), 0.0f);
-            }
+        for (int slot = 0; slot < count; slot++) {  // update_RBC, :204-216
+            random_boundary(rbc, slot, srcId + slot, rbc_ratio, rbc_varVp);
+            check(fwi_shot_get_backward_data(ctx, slot, residual.data()), "fwi_shot_get_backward_data");
             check(fwi_shot_set_backward_data(rbc, slot, residual.data()), "fwi_shot_set_backward_data");
         }
         check(fwi_forward(rbc, FWI_BOUNDARY_RBC), "fwi_forward");  // :218-233
@@ -130,6 +135,13 @@ void Inversion::compute_gradient() {
     if (!final_check) {
         float gmax = 0.0f;
         check(fwi_export_gradient(rbc, zmask, gradient.data(), &gmax), "fwi_export_gradient");  // :142-169
+        if (export_gradients) {
+            const std::string file = output_folder + "gradient_iteration_" + std::to_string(iteration + 1) + "_" +
+                                     std::to_string(int(fmax)) + "Hz_" + std::to_string(nz) + "x" +
+                                     std::to_string(nx) + "x" + std::to_string(ny) + "_" +
+                                     std::to_string((int)(dh)) + "m.bin";
+            export_binary_float(file, gradient.data(), nPoints);
+        }
     }
 }
 
@@ -182,8 +194,19 @@ void Inversion::optimization() {
     const float num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2;
     const float den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2;
     step = 0.5f * (num / den);
-    std::cout << "Line search: f(0) = " << f0 << ", f(a1) = " << f1 << ", f(a2) = " << f2 << ", step = " << step
-              << "\n";
+    // %.9g prints every float exactly (it reads back to the same value)
+    char line[256];
+    std::snprintf(line, sizeof line, "Line search: a1 = %.9g, a2 = %.9g, f(0) = %.9g, f(a1) = %.9g, f(a2) = %.9g, "
+                  "step = %.9g\n", a1, a2, f0, f1, f2, step);
+    std::cout << line;
+    if (!std::isfinite(step)) {
+        // three collinear misfits (den = 0) have no vertex; the reference
+        // would carry the non-finite step into the model, which the library
+        // rejects, so the better of the two trial steps is taken instead
+        step = f1 < f2 ? a1 : a2;
+        std::snprintf(line, sizeof line, "Line search: no parabola vertex, step = %.9g (the better trial)\n", step);
+        std::cout << line;
+    }
 }
 
 // inversion.cu:349-362

@@ -33,6 +33,7 @@ private:
     std::vector<float> trial;     // interior velocity of a line-search trial
     std::vector<float> last_trial;  // velocity of the latest trial (fwi_model_upload = last_trial)
     bool upload_updated = true;     // fwi_model_upload = updated (default) | last_trial
+    bool export_gradients = false;  // inv_export_gradient = true: write each normalised gradient
     std::vector<float> obs_data;
     std::vector<float> residual;
 

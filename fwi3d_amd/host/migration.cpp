@@ -39,10 +39,7 @@ void Migration::forward_propagation() {
     stage_info = "Forward propagation";
     show_mig_info();
     initialization();
-    for (int slot = 0; slot < batch; slot++) {
-        const int shot = first_shot + (slot < shot_count ? slot : 0);
-        random_boundary(ctx, slot, shot, rbc_ratio, rbc_varVp);
-    }
+    for (int slot = 0; slot < shot_count; slot++) random_boundary(ctx, slot, first_shot + slot, rbc_ratio, rbc_varVp);
     forward_solver();
 }
 
@@ -54,15 +51,12 @@ void Migration::backward_propagation() {
     check(fwi_rtm_backward(ctx), "fwi_rtm_backward");
 }
 
-// migration.cu:83-88; unused trailing slots inject zeros
+// migration.cu:83-88 (the unused slots of a last, partial batch are
+// disabled: they have no receivers and inject nothing)
 void Migration::set_seismic_source() {
-    for (int slot = 0; slot < batch; slot++) {
-        if (slot < shot_count) {
-            const std::string data_file = input_folder + input_prefix + std::to_string(first_shot + slot + 1) + ".bin";
-            import_binary_float(data_file, seismogram.data(), (long)nt * geometry->nrec);
-        } else {
-            std::fill(seismogram.begin(), seismogram.end(), 0.0f);
-        }
+    for (int slot = 0; slot < shot_count; slot++) {
+        const std::string data_file = input_folder + input_prefix + std::to_string(first_shot + slot + 1) + ".bin";
+        import_binary_float(data_file, seismogram.data(), (long)nt * geometry->nrec);
         check(fwi_shot_set_backward_data(ctx, slot, seismogram.data()), "fwi_shot_set_backward_data");
     }
 }

@@ -170,8 +170,11 @@ void Modeling::initialization() {
     for (int r = 0; r < geometry->nrec; r++)
         grid_index(geometry->xrec[r], geometry->yrec[r], geometry->zrec[r], &rx[r], &ry[r], &rz[r]);
     for (int slot = 0; slot < batch; slot++) {
-        // unused trailing slots repeat the first shot; their output is dropped
-        const int shot = first_shot + (slot < shot_count ? slot : 0);
+        if (slot >= shot_count) {  // the last batch's unused slots carry no shot
+            check(fwi_shot_disable(ctx, slot), "fwi_shot_disable");
+            continue;
+        }
+        const int shot = first_shot + slot;
         grid_index(geometry->xsrc[shot], geometry->ysrc[shot], geometry->zsrc[shot], &sIdx, &sIdy, &sIdz);
         check(fwi_shot_set_source(ctx, slot, sIdx, sIdy, sIdz), "fwi_shot_set_source");
         check(fwi_shot_set_receivers(ctx, slot, geometry->nrec, rx.data(), ry.data(), rz.data()),

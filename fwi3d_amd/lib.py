@@ -37,7 +37,7 @@ EXPORTS = [
     "fwi_random_boundary_blobs", "fwi_shot_random_boundary_blobs", "fwi_rbc_blobs",
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
-    "fwi_copy_bandwidth",
+    "fwi_copy_bandwidth", "fwi_shot_disable",
 ]
 
 
@@ -140,6 +140,7 @@ def load():
         "fwi_get_shots_per_block": [P, I, ctypes.POINTER(I)],
         "fwi_shot_get_backward_data": [P, I, f32],
         "fwi_copy_bandwidth": [P, ctypes.c_long, I, ctypes.POINTER(ctypes.c_double)],
+        "fwi_shot_disable": [P, I],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -242,6 +243,12 @@ class Context:
 
     def set_source(self, sx, sy, sz, slot=0):
         self._chk(self.L.fwi_shot_set_source(self.h, slot, int(sx), int(sy), int(sz)))
+
+    def disable_slot(self, slot):
+        """No shot in this slot: no source, no receivers, nothing added to the
+        backward pass's sums."""
+        self._chk(self.L.fwi_shot_disable(self.h, slot))
+        self.nrec_slot[slot] = 0
 
     def set_receivers(self, rec, slot=0):
         rec = np.asarray(rec, np.int32).reshape(-1, 3)

@@ -225,6 +225,12 @@ int fwi_shot_random_boundary_blobs(fwi_ctx* ctx, int slot, uint64_t seed, int sh
                                    float vclamp);
 int fwi_shot_set_source(fwi_ctx* ctx, int slot, int sIdx, int sIdy, int sIdz);
 int fwi_shot_set_receivers(fwi_ctx* ctx, int slot, int nrec, const int* rIdx, const int* rIdy, const int* rIdz);
+/* A slot that carries no shot: no source and no receivers, so its fields
+ * stay zero and it adds nothing to the image, illumination or gradient of a
+ * backward pass (the padding of a last, partial batch of shots; the
+ * reference runs shot by shot and has no such slot).  fwi_shot_set_source
+ * enables the slot again. */
+int fwi_shot_disable(fwi_ctx* ctx, int slot);
 int fwi_shot_get_seismogram(fwi_ctx* ctx, int slot, float* out);
 int fwi_shot_set_backward_data(fwi_ctx* ctx, int slot, const float* data);
 int fwi_shot_residual(fwi_ctx* ctx, int slot, const float* obs, double* sum_sq);

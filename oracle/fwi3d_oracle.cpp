@@ -420,53 +420,81 @@ void forward_impl(const float* vp, int nxx, int nyy, int nzz, int nb, float dh, 
 //   4: aligned imaging -- the image and illumination use the current source
 //      field So (the time of the receiver field R) instead of the
 //      reconstructed one.
+// One backward step b: receiver injection into Pr, then the reconstruction
+// Ps <- step(Psold, Ps), the receiver update Prold <- step(Pr, Prold) and the
+// imaging terms added to image / illum (the caller swaps Ps/Psold and
+// Pr/Prold afterwards).
+template <typename T>
+void backward_step(const float* vp, int nxx, int nyy, int nzz, float dh, float dt, int nt, const float* data,
+                   int nrec, const int* rIdx, const int* rIdy, const int* rIdz, int fwi, int b, T* Ps, T* Psold,
+                   T* Pr, T* Prold, T* image, T* illum, int phys, const float* wavelet, long src, int nsteps) {
+    const float idh2f = 1.0f / (dh * dh);
+    const float idh3f = 1.0f / (dh * dh * dh);
+    const T idh2 = (T)idh2f, idh3 = (T)idh3f, tdt = (T)dt;
+    const long sx = nzz, sy = (long)nxx * nzz;
+    if (b < nt)
+        for (int r = 0; r < nrec; r++)
+            Pr[lidx(rIdz[r], rIdx[r], rIdy[r], nxx, nzz)] += idh3 * (T)data[(long)(nt - b - 1) + (long)r * nt];
+    const int ns = nsteps - b;
+    const bool sub = (phys & 2) && src >= 0 && ns >= 0 && ns < nt;
+    const T src_term = sub ? idh3 * (T)wavelet[ns] : (T)0;
+#pragma omp parallel for schedule(static)
+    for (int k = 4; k < nyy - 4; k++)
+        for (int j = 4; j < nxx - 4; j++) {
+            long base = (long)j * sx + (long)k * sy;
+            for (int i = 4; i < nzz - 4; i++) {
+                long c = base + i;
+                T v = (T)vp[c];
+                T vp2 = v * v;
+                T sxx = Deriv<T>::d2(Psold, c, sx, idh2);
+                T syy = Deriv<T>::d2(Psold, c, sy, idh2);
+                T szz = Deriv<T>::d2(Psold, c, 1, idh2);
+                T rxx = Deriv<T>::d2(Pr, c, sx, idh2);
+                T ryy = Deriv<T>::d2(Pr, c, sy, idh2);
+                T rzz = Deriv<T>::d2(Pr, c, 1, idh2);
+                T sn = tdt * tdt * vp2 * (sxx + syy + szz) + (T)2 * Psold[c] - Ps[c];
+                if (sub && c == src) sn = sn + src_term;
+                Ps[c] = sn;
+                Prold[c] = tdt * tdt * vp2 * (rxx + ryy + rzz) + (T)2 * Pr[c] - Prold[c];
+                const T si = (phys & 4) ? Psold[c] : Ps[c];
+                illum[c] += si * si;
+                if (fwi)
+                    image[c] += tdt * Pr[c] * ((phys & 1) ? sxx + syy + szz : sxx + szz) * vp2;  // inversion.cu:466
+                else
+                    image[c] += si * Pr[c];  // migration.cu:200
+            }
+        }
+}
+
+// Backward pass: time-reversed source reconstruction + receiver-side
+// propagation + imaging (cross_correlation, migration.cu:134-202) or
+// gradient (build_gradient, inversion.cu:400-468), with the receiver
+// injection of inject_seismogram / inject_residuo (migration.cu:119-132,
+// inversion.cu:385-398) applied in receiver order before the step.
+//
+// phys (0 = the reference) switches on corrections the reference lacks:
+//   1: FWI gradient with d2/dy2 (inversion.cu:466 omits it);
+//   2: source subtraction -- the newer source-side field of backward step b
+//      is the forward field of step nsteps - b, which carries the source term
+//      idh3 * wavelet[nsteps - b] (for nsteps - b < nt); it is added back to
+//      the reconstructed field at the source cell, which makes the
+//      reconstruction the exact reverse of the forward recurrence;
+//   4: aligned imaging -- the image and illumination use the current source
+//      field So (the time of the receiver field R) instead of the
+//      reconstructed one.
 template <typename T>
 void backward_impl(const float* vp, int nxx, int nyy, int nzz, float dh, float dt, int nt, const float* data,
                    int nrec, const int* rIdx, const int* rIdy, const int* rIdz, int fwi, int b_begin, int b_end,
                    T* S, T* So, T* R, T* Ro, T* image, T* illum, int phys = 0, const float* wavelet = nullptr,
                    long src = -1, int nsteps = 0) {
     Dims d{nxx, nyy, nzz};
-    const float idh2f = 1.0f / (dh * dh);
-    const float idh3f = 1.0f / (dh * dh * dh);
-    const T idh2 = (T)idh2f, idh3 = (T)idh3f, tdt = (T)dt;
-    const long sx = nzz, sy = (long)nxx * nzz;
     T* Ps = S;
     T* Psold = So;
     T* Pr = R;
     T* Prold = Ro;
     for (int b = b_begin; b < b_end; b++) {
-        if (b < nt)
-            for (int r = 0; r < nrec; r++)
-                Pr[lidx(rIdz[r], rIdx[r], rIdy[r], nxx, nzz)] += idh3 * (T)data[(long)(nt - b - 1) + (long)r * nt];
-        const int ns = nsteps - b;
-        const bool sub = (phys & 2) && src >= 0 && ns >= 0 && ns < nt;
-        const T src_term = sub ? idh3 * (T)wavelet[ns] : (T)0;
-#pragma omp parallel for schedule(static)
-        for (int k = 4; k < nyy - 4; k++)
-            for (int j = 4; j < nxx - 4; j++) {
-                long base = (long)j * sx + (long)k * sy;
-                for (int i = 4; i < nzz - 4; i++) {
-                    long c = base + i;
-                    T v = (T)vp[c];
-                    T vp2 = v * v;
-                    T sxx = Deriv<T>::d2(Psold, c, sx, idh2);
-                    T syy = Deriv<T>::d2(Psold, c, sy, idh2);
-                    T szz = Deriv<T>::d2(Psold, c, 1, idh2);
-                    T rxx = Deriv<T>::d2(Pr, c, sx, idh2);
-                    T ryy = Deriv<T>::d2(Pr, c, sy, idh2);
-                    T rzz = Deriv<T>::d2(Pr, c, 1, idh2);
-                    T sn = tdt * tdt * vp2 * (sxx + syy + szz) + (T)2 * Psold[c] - Ps[c];
-                    if (sub && c == src) sn = sn + src_term;
-                    Ps[c] = sn;
-                    Prold[c] = tdt * tdt * vp2 * (rxx + ryy + rzz) + (T)2 * Pr[c] - Prold[c];
-                    const T si = (phys & 4) ? Psold[c] : Ps[c];
-                    illum[c] += si * si;
-                    if (fwi)
-                        image[c] += tdt * Pr[c] * ((phys & 1) ? sxx + syy + szz : sxx + szz) * vp2;  // inversion.cu:466
-                    else
-                        image[c] += si * Pr[c];  // migration.cu:200
-                }
-            }
+        backward_step<T>(vp, nxx, nyy, nzz, dh, dt, nt, data, nrec, rIdx, rIdy, rIdz, fwi, b, Ps, Psold, Pr, Prold,
+                         image, illum, phys, wavelet, src, nsteps);
         std::swap(Ps, Psold);
         std::swap(Pr, Prold);
     }
@@ -547,6 +575,42 @@ void oracle_backward_phys_f64(const float* vp, int nxx, int nyy, int nzz, float 
                               int nsteps) {
     backward_impl<double>(vp, nxx, nyy, nzz, dh, dt, nt, data, nrec, rIdx, rIdy, rIdz, fwi, b_begin, b_end, S, So,
                           R, Ro, image, illum, phys, wavelet, lidx(sIdz, sIdx, sIdy, nxx, nzz), nsteps);
+}
+
+// A batch of nshots shots advanced step by step, the shots in slot order
+// within every step, shot s adding its terms into the image / illumination
+// of group s / sb (the summation order of a batched backward pass with sb
+// shots per block; the caller folds the groups in group order afterwards).
+// Per shot: vps[s], datas[s] (nrecs[s] x nt), receivers rx/ry/rz[s], source
+// src[3s..3s+2] (x, y, z), fields S/So/R/Ro[s], handed back as (current,
+// previous) like oracle_backward_f32.  Steps [b_begin, b_end); phys as in
+// backward_impl.
+void oracle_backward_batched_f32(const float* const* vps, int nshots, int sb, int nxx, int nyy, int nzz, float dh,
+                                 float dt, int nt, const float* const* datas, const int* nrecs,
+                                 const int* const* rIdx, const int* const* rIdy, const int* const* rIdz, int fwi,
+                                 int b_begin, int b_end, float* const* S, float* const* So, float* const* R,
+                                 float* const* Ro, float* const* image, float* const* illum, int phys,
+                                 const float* wavelet, const int* src, int nsteps) {
+    std::vector<float*> ps(S, S + nshots), pso(So, So + nshots), pr(R, R + nshots), pro(Ro, Ro + nshots);
+    for (int b = b_begin; b < b_end; b++)
+        for (int s = 0; s < nshots; s++) {
+            const long sc = src[3 * s] >= 0 ? lidx(src[3 * s + 2], src[3 * s], src[3 * s + 1], nxx, nzz) : -1;
+            backward_step<float>(vps[s], nxx, nyy, nzz, dh, dt, nt, datas[s], nrecs[s], rIdx[s], rIdy[s], rIdz[s],
+                                 fwi, b, ps[s], pso[s], pr[s], pro[s], image[s / sb], illum[s / sb], phys, wavelet,
+                                 sc, nsteps);
+            std::swap(ps[s], pso[s]);
+            std::swap(pr[s], pro[s]);
+        }
+    const long n = (long)nxx * nyy * nzz;
+    if ((b_end - b_begin) % 2 == 1 && b_end > b_begin)
+        for (int s = 0; s < nshots; s++) {
+            std::vector<float> tmp(S[s], S[s] + n);
+            std::memcpy(S[s], So[s], n * sizeof(float));
+            std::memcpy(So[s], tmp.data(), n * sizeof(float));
+            tmp.assign(R[s], R[s] + n);
+            std::memcpy(R[s], Ro[s], n * sizeof(float));
+            std::memcpy(Ro[s], tmp.data(), n * sizeof(float));
+        }
 }
 
 // ---------------------------------------------------------------------------

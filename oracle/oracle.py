@@ -65,6 +65,10 @@ def lib():
     phys = [_c_int, _f32p, _c_int, _c_int, _c_int, _c_int]
     L.oracle_backward_phys_f32.argtypes = bwd + [_f32p] * 6 + phys
     L.oracle_backward_phys_f64.argtypes = bwd + [_f64p] * 6 + phys
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    L.oracle_backward_batched_f32.argtypes = [pp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_float, _c_float, _c_int,
+                                              pp, _i32p, pp, pp, pp, _c_int, _c_int, _c_int, pp, pp, pp, pp, pp, pp,
+                                              _c_int, _f32p, _i32p, _c_int]
     L.oracle_export_seismic.argtypes = [_f32p, _f32p] + [_c_int] * 4 + [_c_float, _f32p]
     L.oracle_gradient_post.argtypes = [_f32p, _f32p] + [_c_int] * 4 + [_c_float, _c_float, _f32p]
     L.oracle_gradient_post.restype = _c_float
@@ -232,6 +236,42 @@ def backward(vp, dh, dt, nt, data, rec, S, So, R=None, Ro=None, image=None, illu
         fn = lib().oracle_backward_f64 if fp64 else lib().oracle_backward_f32
         fn(*args)
     return S, So, R, Ro, image, illum
+
+
+def _ptrs(arrays):
+    return (ctypes.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
+def backward_batched(vps, sb, dh, dt, nt, datas, recs, states, images, illums, fwi=False, b_begin=0, b_end=None,
+                     tlag=0, phys=0, wavelet_=None, srcs=None):
+    """Backward steps [b_begin, b_end) of a batch of shots in the summation
+    order of a batched pass with sb shots per block: every step advances the
+    shots in slot order and shot s adds its terms into group s // sb.
+    states[s] = [S, So, R, Ro] float32 arrays, updated in place; images /
+    illums: one float32 array per group, updated in place."""
+    S = len(vps)
+    vps = [np.ascontiguousarray(v, np.float32) for v in vps]
+    nyy, nxx, nzz = vps[0].shape
+    b_end = nt + tlag if b_end is None else b_end
+    datas = [np.ascontiguousarray(d, np.float32) for d in datas]
+    recs = [np.asarray(r, np.int32).reshape(-1, 3) for r in recs]
+    rx = [np.ascontiguousarray(r[:, 0]) for r in recs]
+    ry = [np.ascontiguousarray(r[:, 1]) for r in recs]
+    rz = [np.ascontiguousarray(r[:, 2]) for r in recs]
+    for st in states:
+        for a in st:
+            assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == vps[0].shape
+    for a in list(images) + list(illums):
+        assert a.dtype == np.float32 and a.flags.c_contiguous and a.shape == vps[0].shape
+    assert len(images) == len(illums) == (S + sb - 1) // sb
+    nrecs = np.array([len(r) for r in recs], np.int32)
+    src = np.full(3 * S, -1, np.int32) if srcs is None else np.ascontiguousarray(np.asarray(srcs, np.int32).ravel())
+    w = np.ascontiguousarray(wavelet_ if wavelet_ is not None else np.zeros(nt), np.float32)
+    lib().oracle_backward_batched_f32(_ptrs(vps), S, sb, nxx, nyy, nzz, dh, dt, nt, _ptrs(datas), nrecs,
+                                      _ptrs(rx), _ptrs(ry), _ptrs(rz), 1 if fwi else 0, b_begin, b_end,
+                                      _ptrs([st[0] for st in states]), _ptrs([st[1] for st in states]),
+                                      _ptrs([st[2] for st in states]), _ptrs([st[3] for st in states]),
+                                      _ptrs(list(images)), _ptrs(list(illums)), int(phys), w, src, int(nt + tlag))
 
 
 def export_seismic(image_pad, illum_pad, nb, dh):

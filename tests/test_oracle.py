@@ -32,6 +32,10 @@ function is checked against what the mathematics fixes:
   model_step                 1/sqrt(m - a g) in FP64, rounded
   random boundary            interior untouched; |v/v_face - 1| <= amp*ramp;
                              clamp; seeded determinism; U[-1,1) moments
+  backward_batched           one shot = the single-shot backward; one shot per
+                             group = per-shot backward passes summed in group
+                             order; sb > 1 = the step-by-step interleave of
+                             single-step backward calls (the batched order)
   blob random boundary       exp_det = math.exp to 1e-15; Irwin-Hall normal
                              moments and bounds; one blob on a constant model
                              = v0 + A e^(-q/2) cut at 4 radii (closed form);
@@ -433,3 +437,75 @@ def test_blob_boundary_restarts_from_the_interior():
     assert np.array_equal(a, b)
     assert np.array_equal(a[nb:-nb, nb:-nb, nb:-nb], vp[nb:-nb, nb:-nb, nb:-nb])
     assert not np.array_equal(a, vp)
+
+
+def _batched_case(S=3, seed=5):
+    from tests.cases import Case
+    case = Case(nx=20, ny=18, nz=16, nb=4, nt=24, amp=0.0)
+    rng = np.random.default_rng(seed)
+    pad = O.expand_boundary(case.interior, case.nb)
+    vps = [O.random_boundary(pad, case.nb, 9, s, 0.5, case.vclamp) for s in range(S)]
+    shots = []
+    for s in range(S):
+        src = (case.nb + 3 + 3 * s, case.nb + 5, case.nb + 2)
+        rec = case.rec[s::S] if s != 1 else case.rec[:0]  # one shot without receivers
+        shots.append((src, rec))
+    states, datas = [], []
+    for s, (src, rec) in enumerate(shots):
+        P, Pold, _ = O.forward(vps[s], case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, src, rec,
+                               record=False)
+        states.append([P, Pold, np.zeros_like(P), np.zeros_like(P)])
+        datas.append(rng.normal(0, 1e-2, (len(rec), case.nt)).astype(np.float32))
+    return case, vps, shots, states, datas
+
+
+def _copy_states(states):
+    return [[a.copy() for a in st] for st in states]
+
+
+@pytest.mark.parametrize("fwi,phys", [(False, 0), (True, 0), (True, 7)])
+def test_backward_batched_orders(fwi, phys):
+    case, vps, shots, states0, datas = _batched_case()
+    S = len(vps)
+    kw = dict(fwi=fwi, tlag=case.tlag, phys=phys, wavelet_=case.wavelet)
+    srcs = [src for src, _ in shots]
+    zero = np.zeros_like(vps[0])
+    # one shot: the single-shot backward
+    st = _copy_states(states0[:1])
+    img, ill = [zero.copy()], [zero.copy()]
+    O.backward_batched(vps[:1], 1, case.dh, case.dt, case.nt, datas[:1], [shots[0][1]], st, img, ill, srcs=srcs[:1],
+                       **kw)
+    want = O.backward(vps[0], case.dh, case.dt, case.nt, datas[0], shots[0][1], *states0[0], fwi=fwi, tlag=case.tlag,
+                      phys=phys, wavelet_=case.wavelet, src=srcs[0])
+    for a, b in zip(st[0] + [img[0], ill[0]], want):
+        assert np.array_equal(a, b)
+    # one shot per group: independent per-shot passes, summed in group order
+    st = _copy_states(states0)
+    img, ill = [zero.copy() for _ in range(S)], [zero.copy() for _ in range(S)]
+    O.backward_batched(vps, 1, case.dh, case.dt, case.nt, datas, [r for _, r in shots], st, img, ill, srcs=srcs, **kw)
+    for s in range(S):
+        want = O.backward(vps[s], case.dh, case.dt, case.nt, datas[s], shots[s][1], *states0[s], fwi=fwi,
+                          tlag=case.tlag, phys=phys, wavelet_=case.wavelet, src=srcs[s])
+        for a, b in zip(st[s] + [img[s], ill[s]], want):
+            assert np.array_equal(a, b)
+    # sb = 2 over an odd segment: the step-by-step interleave of single steps
+    b0, b1 = 3, 3 + 17
+    st = _copy_states(states0)
+    img, ill = [zero.copy(), zero.copy()], [zero.copy(), zero.copy()]
+    O.backward_batched(vps, 2, case.dh, case.dt, case.nt, datas, [r for _, r in shots], st, img, ill,
+                       b_begin=b0, b_end=b1, srcs=srcs, **kw)
+    ref = _copy_states(states0)
+    rimg, rill = [zero.copy(), zero.copy()], [zero.copy(), zero.copy()]
+    for b in range(b0, b1):
+        for s in range(S):
+            out = O.backward(vps[s], case.dh, case.dt, case.nt, datas[s], shots[s][1], *ref[s], rimg[s // 2],
+                             rill[s // 2], fwi=fwi, b_begin=b, b_end=b + 1, tlag=case.tlag, phys=phys,
+                             wavelet_=case.wavelet, src=srcs[s])
+            ref[s] = list(out[:4])
+            rimg[s // 2], rill[s // 2] = out[4], out[5]
+    assert np.abs(rimg[0]).max() > 0 and np.abs(rill[1]).max() > 0
+    for s in range(S):
+        for a, b in zip(st[s], ref[s]):
+            assert np.array_equal(a, b)
+    for g in range(2):
+        assert np.array_equal(img[g], rimg[g]) and np.array_equal(ill[g], rill[g])
