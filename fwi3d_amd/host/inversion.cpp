@@ -199,12 +199,27 @@ void Inversion::optimization() {
     std::snprintf(line, sizeof line, "Line search: a1 = %.9g, a2 = %.9g, f(0) = %.9g, f(a1) = %.9g, f(a2) = %.9g, "
                   "step = %.9g\n", a1, a2, f0, f1, f2, step);
     std::cout << line;
-    if (!std::isfinite(step)) {
-        // three collinear misfits (den = 0) have no vertex; the reference
-        // would carry the non-finite step into the model, which the library
-        // rejects, so the better of the two trial steps is taken instead
-        step = f1 < f2 ? a1 : a2;
-        std::snprintf(line, sizeof line, "Line search: no parabola vertex, step = %.9g (the better trial)\n", step);
+    // A vertex the reference would take but whose model the library cannot
+    // run: a non-finite step (three collinear misfits, den = 0), or a step
+    // that drives some slowness squared to zero or below (NaN velocities in
+    // the reference) or a velocity past the stability limit.  The better of
+    // the two trial steps (or no step) is taken instead; otherwise the
+    // reference's vertex stands.
+    const float vcfl = cfl_velocity();
+    auto usable = [&](float a) {
+        if (!std::isfinite(a)) return false;
+        for (long q = 0; q < nPoints; q++) {
+            const float mm = m[q] - a * gradient[q];
+            if (!(mm > 0.0f) || !(1.0f / sqrtf(mm) <= vcfl)) return false;
+        }
+        return true;
+    };
+    if (!usable(step)) {
+        const float better = f1 < f2 ? a1 : a2;
+        step = usable(better) ? better : 0.0f;
+        std::snprintf(line, sizeof line,
+                      "Line search: the vertex leaves the valid model range; step = %.9g (the better trial step)\n",
+                      step);
         std::cout << line;
     }
 }

@@ -164,26 +164,36 @@ def _run(exe, params):
     return r.stdout
 
 
-def test_config3_inversion_iteration_vs_oracle(fl, tmp_path):
+@pytest.mark.parametrize("case", ["vertex", "guarded"])
+def test_config3_inversion_iteration_vs_oracle(fl, tmp_path, case):
     """One inversion_main iteration (16 shots, batch_shots = 1, reference-style
-    blob boundary) on a 40 x 40 x 24 gradient model from a smoothed initial
-    model: the first misfit, the exported normalised gradient, the two
-    line-search misfits, the parabola step (inversion.cu:291-312) and the
-    written model (inversion.cu:349-362) against the oracle."""
+    blob boundary) on a shrunken gradient model from a smoothed initial model:
+    the first misfit, the exported normalised gradient, the two line-search
+    misfits, the parabola step (inversion.cu:291-312) and the written model
+    (inversion.cu:349-362) against the oracle.  "vertex": 40 x 40 x 32,
+    depth mask 100 m, the parabola's vertex gives a usable model and is
+    taken, as in the reference.  "guarded": 40 x 40 x 24, depth mask 30 m, the
+    vertex lies 25 trial steps out and would drive the slowness negative
+    (NaN velocities in the reference); the driver takes the better trial
+    step instead."""
     from scipy.ndimage import gaussian_filter
     from fwi3d_amd import configs
-    w = configs.gradient_survey(nx=40, ny=40, nz=24, nb=10, nt=200, full_spread=True, spacing=40.0)
+    nz, nt, sigma, zmask = (32, 250, 1.0, 100.0) if case == "vertex" else (24, 200, 2.0, 30.0)
+    w = configs.gradient_survey(nx=40, ny=40, nz=nz, nb=10, nt=nt, full_spread=True, spacing=40.0)
     folder = str(tmp_path / "fwi3")
     _run("modeling_main", w.write(folder))  # observed data, absorbing boundary
-    init = (1.0 / gaussian_filter(1.0 / w.interior.astype(np.float64), 4.0)).astype(np.float32)
-    params = w.write(folder, model=init, kind="init", max_iteration=1, batch_shots=1, depth_mask=30.0)
+    init = (1.0 / gaussian_filter(1.0 / w.interior.astype(np.float64), sigma)).astype(np.float32)
+    params = w.write(folder, model=init, kind="init", max_iteration=1, batch_shots=1, depth_mask=zmask)
     with open(params, "a") as f:
         f.write("inv_export_gradient = true\n")
     out = _run("inversion_main", params)
     line = re.search(r"Line search: a1 = (\S+), a2 = (\S+), f\(0\) = (\S+), f\(a1\) = (\S+), f\(a2\) = (\S+), "
                      r"step = (\S+)\n", out)
     assert line, out[-2000:]
-    a1, a2, f0, f1, f2, step = (np.float32(float(v)) for v in line.groups())
+    a1, a2, f0, f1, f2, vertex = (np.float32(float(v)) for v in line.groups())
+    guard = re.search(r"Line search: the vertex leaves the valid model range; step = (\S+) ", out)
+    step = np.float32(float(guard.group(1))) if guard else vertex
+    assert (guard is not None) == (case == "guarded"), out[-1500:]
     dims = "%dx%dx%d_%dm.bin" % (w.nz, w.nx, w.ny, int(w.dh))
     g_file = np.fromfile(os.path.join(folder, "outputs/models", "gradient_iteration_1_60Hz_" + dims), np.float32)
     m_file = np.fromfile(os.path.join(folder, "outputs/models", "model_FWI_iteration_1_60Hz_" + dims), np.float32)
@@ -227,7 +237,7 @@ def test_config3_inversion_iteration_vs_oracle(fl, tmp_path):
         out_b = O.backward(vp, w.dh, w.dt, w.nt, res0[s], rec, P, Pold, image=grad, illum=illum, fwi=True,
                            tlag=w.tlag)
         grad, illum = out_b[4], out_b[5]
-    g, _ = O.gradient_post(grad, illum, nb, w.dh, 30.0)
+    g, _ = O.gradient_post(grad, illum, nb, w.dh, zmask)
     assert np.abs(g).max() > 0
     assert np.array_equal(g_file, g.ravel())
     # line search (inversion.cu:291-347)
@@ -238,9 +248,16 @@ def test_config3_inversion_iteration_vs_oracle(fl, tmp_path):
         _, vt = O.model_step(m, g, a)
         st, _ = misfit(vt.reshape(m.shape))
         assert abs(float(np.sqrt(st)) - f) <= 1e-6 * f, (a, np.sqrt(st), f)
-    want_step = O.parabola_step(f32(0.0), a1, a2, f0, f1, f2)
-    assert np.isfinite(want_step) and f32(want_step) == step, (want_step, step)
-    _, v1 = O.model_step(m, g, step)
+    want_vertex = O.parabola_step(f32(0.0), a1, a2, f0, f1, f2)
+    assert np.isfinite(want_vertex) and f32(want_vertex) == vertex, (want_vertex, vertex)
+    mo, v1 = O.model_step(m, g, vertex)
+    if case == "guarded":  # the vertex model is unusable; the better trial is taken
+        assert (mo <= 0).any() or not np.isfinite(v1).all() or v1.max() > vclamp
+        assert step == (a1 if f1 < f2 else a2)
+        _, v1 = O.model_step(m, g, step)
+    assert np.isfinite(v1).all() and v1.max() <= vclamp
     assert np.array_equal(m_file, v1)
     conv = np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_1_iterations.txt"))
     assert conv.shape == (2,) and conv[1] < conv[0]
+    s1, _ = misfit(v1.reshape(m.shape))
+    assert abs(float(np.sqrt(s1)) - conv[1]) <= 1e-5 * conv[1]
