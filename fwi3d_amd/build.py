@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-HIP_SOURCES = ["abi.hip", "kernels_step.hip", "kernels_aux.hip", "unet_kernels.hip", "unet_net.hip"]
+HIP_SOURCES = ["abi.hip", "kernels_step.hip", "kernels_aux.hip", "unet_kernels.hip", "unet_net.hip",
+               "noalloc.hip"]
 
 CXX = shutil.which("g++") or "g++"
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-ffp-contract=off",

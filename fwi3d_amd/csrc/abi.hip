@@ -594,6 +594,11 @@ std::vector<float> make_blobs(uint64_t seed, int shot, int nxx, int nyy, int nzz
 
 }  // namespace
 
+namespace fwi {
+// the message fwi_last_error(NULL) returns (entry points without a context)
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace fwi
+
 extern "C" {
 
 int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out) {

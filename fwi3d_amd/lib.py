@@ -37,7 +37,7 @@ EXPORTS = [
     "fwi_random_boundary_blobs", "fwi_shot_random_boundary_blobs", "fwi_rbc_blobs",
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
-    "fwi_copy_bandwidth", "fwi_shot_disable",
+    "fwi_copy_bandwidth", "fwi_shot_disable", "fwi_residual_noalloc",
 ]
 
 
@@ -69,6 +69,17 @@ def rbc_blobs(seed, shot, nxx, nyy, nzz, nb, dh, ratio, var_vp):
         if rc != OK:
             raise FwiError(rc, "fwi_rbc_blobs")
     return out[:n.value].copy()
+
+
+def residual_noalloc(obs_ptr, cal_ptr, n, sum_sq_ptr, stream=None):
+    """fwi_residual_noalloc on device addresses (ints): cal <- obs - cal and
+    *sum_sq_ptr <- sum r^2 (FP64), asynchronous on `stream` (a hipStream_t
+    handle as an int, or None for the null stream); allocates nothing."""
+    L = load()
+    rc = L.fwi_residual_noalloc(ctypes.c_void_p(obs_ptr), ctypes.c_void_p(cal_ptr), int(n),
+                                ctypes.c_void_p(sum_sq_ptr), ctypes.c_void_p(stream or 0))
+    if rc != OK:
+        raise FwiError(rc, L.fwi_last_error(None).decode())
 
 
 class Grid(ctypes.Structure):
@@ -141,6 +152,7 @@ def load():
         "fwi_shot_get_backward_data": [P, I, f32],
         "fwi_copy_bandwidth": [P, ctypes.c_long, I, ctypes.POINTER(ctypes.c_double)],
         "fwi_shot_disable": [P, I],
+        "fwi_residual_noalloc": [P, P, ctypes.c_long, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -306,6 +306,22 @@ int fwi_copy_bandwidth(fwi_ctx* ctx, long bytes, int reps, double* gbps);
 /* Synchronise the context's stream. */
 int fwi_sync(fwi_ctx* ctx);
 
+/* ---- Allocation-free variant ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight loops, hosts that forbid
+ * hipMalloc): the residual of fwi_residual on caller device buffers, with no
+ * context and no allocation.  obs and cal hold n floats in device memory;
+ * cal is overwritten with r = obs - cal and *sum_sq (a device double)
+ * receives sum r^2 reduced in FP64, with the same kernels, block count and
+ * summation order as fwi_residual (equal inputs in the same order give equal
+ * bits).  Runs asynchronously on `stream` (a hipStream_t; NULL = the null
+ * stream) of the current device.  The per-block partial sums use a fixed
+ * 8 KiB device array of the library, one per device, kept until the process
+ * ends: calls that overlap on one device race.  n must be in
+ * [1, FWI_NOALLOC_MAX_SAMPLES]; fwi_last_error(NULL) explains a failure. */
+#define FWI_NOALLOC_MAX_SAMPLES (1L << 34)
+int fwi_residual_noalloc(const float* obs, float* cal, long n, double* sum_sq, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
