@@ -136,6 +136,110 @@ def cpu_vs_gpu_two_layer(lib, configs):
             "seismogram_bit_exact": bool(np.array_equal(g, seis))}
 
 
+def _rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def fp64_errors(lib, configs, w):
+    """Error of the GPU's FP32 fields against an FP64 solution of the same
+    scheme (the oracle's double instantiation: double fields, the same float
+    velocity model), both arithmetic modes:
+      - the full two-layer 101^3 forward modeling (1000 x 6561 seismogram);
+      - the 401x401x201 workload: forward steps [0, 160) then RTM steps
+        [0, 64) (fields, image, illumination);
+      - a whole 3000-sample pass (3074 steps of forward, then 3074 of
+        time-reversed reconstruction and imaging) with this workload's dt,
+        wavelet and model family on a 101 x 101 x 51 grid with a 20-cell
+        boundary: seismogram, image and illumination after the full pass."""
+    import oracle as O
+    out = {}
+    arith = {"reference": lib.ARITH_REFERENCE, "fp32": lib.ARITH_FP32}
+    t0 = time.perf_counter()
+    # two-layer 101^3, full run
+    w0 = configs.two_layer_101()
+    vp0 = O.random_boundary(O.expand_boundary(w0.interior, w0.nb), w0.nb, w0.rbc_seed, 0, w0.rbc_amp, w0.vclamp)
+    _, _, s64 = O.forward(vp0, w0.nb, w0.dh, w0.dt, w0.nt, w0.tlag, w0.wavelet, w0.src, w0.rec, fp64=True)
+    r = {}
+    for name, a in arith.items():
+        c = lib.Context(w0.nxx, w0.nyy, w0.nzz, w0.nb, w0.dh, w0.dt, w0.nt, w0.tlag)
+        c.set_arith(a)
+        c.set_model_interior(w0.interior)
+        c.random_boundary(w0.rbc_seed, 0, w0.rbc_amp, w0.vclamp)
+        c.set_wavelet(w0.wavelet)
+        c.set_source(*w0.src)
+        c.set_receivers(w0.rec)
+        c.forward(lib.BOUNDARY_RBC)
+        r[name] = _rel_l2(c.seismogram(), s64)
+        c.close()
+    out["two_layer_101_seismogram"] = r
+    del s64
+    # full-size workload: forward [0, 160) and RTM [0, 64)
+    nf, nbk = 160, 64
+    vp = O.random_boundary(O.expand_boundary(w.interior, w.nb), w.nb, w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+    P, Pold, seis = O.forward(vp, w.nb, w.dh, w.dt, w.nt, w.tlag, w.wavelet, w.src, w.rec, n_end=nf, fp64=True)
+    rng = np.random.default_rng(1)
+    data = rng.standard_normal((len(w.rec), w.nt), dtype=np.float32) * np.float32(1e-3)
+    S, _, R, _, image, illum = O.backward(vp, w.dh, w.dt, w.nt, data, w.rec, P, Pold, b_begin=0, b_end=nbk,
+                                          tlag=w.tlag, fp64=True)
+    r = {}
+    for name, a in arith.items():
+        c = lib.Context(w.nxx, w.nyy, w.nzz, w.nb, w.dh, w.dt, w.nt, w.tlag)
+        c.set_arith(a)
+        c.set_model_interior(w.interior)
+        c.random_boundary(w.rbc_seed, 0, w.rbc_amp, w.vclamp)
+        c.set_wavelet(w.wavelet)
+        c.set_source(*w.src)
+        c.set_receivers(w.rec)
+        c.forward_steps(lib.BOUNDARY_RBC, 0, nf)
+        e = {"forward_field": _rel_l2(c.get_volume(lib.VOL_FIELD), P),
+             "seismogram": _rel_l2(c.seismogram()[:, :nf - w.tlag], seis[:, :nf - w.tlag])}
+        c.set_backward_data(data)
+        for v in (lib.VOL_RECV, lib.VOL_RECV_OLD, lib.VOL_IMAGE, lib.VOL_ILLUM):
+            c.zero_volume(v)
+        c.backward_steps(0, 0, nbk)
+        e.update({"source_field": _rel_l2(c.get_volume(lib.VOL_FIELD), S),
+                  "receiver_field": _rel_l2(c.get_volume(lib.VOL_RECV), R),
+                  "image": _rel_l2(c.get_volume(lib.VOL_IMAGE), image),
+                  "illumination": _rel_l2(c.get_volume(lib.VOL_ILLUM), illum)})
+        r[name] = e
+        c.close()
+    out["gradient_401_forward_160_rtm_64_steps"] = r
+    del P, Pold, seis, S, R, image, illum, data
+    # whole pass, shrunken grid
+    ws = configs.gradient_survey(nx=101, ny=101, nz=51, nb=20, nt=w.nt, full_spread=True)
+    vps = O.random_boundary(O.expand_boundary(ws.interior, ws.nb), ws.nb, ws.rbc_seed, 0, ws.rbc_amp, ws.vclamp)
+    src = ws.src_index(5)  # an inner shot of the 4 x 4 grid
+    rec = ws.rec_index(5)
+    P, Pold, s64 = O.forward(vps, ws.nb, ws.dh, ws.dt, ws.nt, ws.tlag, ws.wavelet, src, rec, fp64=True)
+    d32 = s64.astype(np.float32)
+    _, _, _, _, image, illum = O.backward(vps, ws.dh, ws.dt, ws.nt, d32, rec, P, Pold, tlag=ws.tlag, fp64=True)
+    r = {}
+    for name, a in arith.items():
+        c = lib.Context(ws.nxx, ws.nyy, ws.nzz, ws.nb, ws.dh, ws.dt, ws.nt, ws.tlag)
+        c.set_arith(a)
+        c.set_model_interior(ws.interior)
+        c.random_boundary(ws.rbc_seed, 0, ws.rbc_amp, ws.vclamp)
+        c.set_wavelet(ws.wavelet)
+        c.set_source(*src)
+        c.set_receivers(rec)
+        c.forward(lib.BOUNDARY_RBC)
+        e = {"seismogram": _rel_l2(c.seismogram(), s64)}
+        c.set_backward_data(d32)
+        c.rtm_backward()
+        e.update({"image": _rel_l2(c.get_volume(lib.VOL_IMAGE), image),
+                  "illumination": _rel_l2(c.get_volume(lib.VOL_ILLUM), illum)})
+        r[name] = e
+        c.close()
+    out["gradient_101x101x51_full_pass"] = r
+    out["note"] = ("relative L2 of the GPU FP32 result against the FP64 oracle; the backward data of the "
+                   "full pass is the FP64 seismogram rounded to float")
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    log("FP64 errors: %s" % json.dumps(out))
+    return out
+
+
 def blob_boundary(w, ctx, lib, shots=3, ratio=100.0, var_vp=1000.0):
     """The reference-style random boundary (modeling.cu:558-673) on the
     workload grid with the reference parameters.txt's mig_rbc_ratio = 100 m
@@ -345,6 +449,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="capture each pass in a hipGraph")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP32-vs-FP64 error report (CPU baseline leg)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pass-steps", type=int, default=0,
                     help="profiling only: cut each pass to this many time steps")
@@ -534,6 +639,9 @@ def main():
         cpu = cpu_baseline(w, vp_pad, args.cpu_seconds)
         log("two-layer 101^3 forward modeling: GPU and CPU oracle ...")
         cpu["two_layer_101_full_run"] = cpu_vs_gpu_two_layer(lib, configs)
+        if not args.no_fp64:
+            log("FP32 error against the FP64 oracle ...")
+            cpu["fp64_error"] = fp64_errors(lib, configs, w)
     else:
         ctx.close()
 

@@ -5,7 +5,7 @@
 //   forward  (compute_pressure, src/modeling/modeling.cu:324-382)
 //       new = dt^2 vp^2 L(cur) + 2 cur - old            [+ Cerjan damping]
 //   backward (cross_correlation, src/migration/migration.cu:181-202, and
-//             build_gradient, src/inversion/inversion.cu:491-558)
+//             build_gradient, src/inversion/inversion.cu:400-468)
 //       S   = dt^2 vp^2 L(So) + 2 So - S   (time-reversed source field)
 //       Ro  = dt^2 vp^2 L(R)  + 2 R  - Ro  (receiver field)
 //       illum += S^2;  image += S R   |  grad += dt R (Dxx + Dzz)(So) vp^2

@@ -6,14 +6,6 @@
 #include <iostream>
 #include <sstream>
 
-// src/admin/admin.cpp:3-11
-bool str2bool(std::string s) {
-    bool b = false;
-    std::for_each(s.begin(), s.end(), [](char& c) { c = (char)::tolower(c); });
-    std::istringstream(s) >> std::boolalpha >> b;
-    return b;
-}
-
 // src/admin/admin.cpp:13-24 (plus a short-read check)
 void import_binary_float(std::string path, float* array, long n) {
     std::ifstream file(path, std::ios::in | std::ios::binary);

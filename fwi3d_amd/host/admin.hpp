@@ -13,8 +13,6 @@
 #include <string>
 #include <vector>
 
-bool str2bool(std::string s);
-
 void import_binary_float(std::string path, float* array, long n);
 void export_binary_float(std::string path, const float* array, long n);
 

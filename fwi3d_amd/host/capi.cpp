@@ -1,6 +1,8 @@
 // capi.cpp -- C entry points into the host library for tests: the host-side
 // set-up arithmetic (wavelet, dampers, parameter parsing) that feeds the
-// device library, and the data-preparation steps of dataset.hpp.
+// device library, run through the Modeling class's own methods, and the
+// data-preparation steps of dataset.hpp.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -8,24 +10,40 @@
 
 #include "admin.hpp"
 #include "dataset.hpp"
+#include "modeling.hpp"
+
+namespace {
+// Runs the drivers' own set-up methods (Modeling::set_wavelet,
+// Modeling::set_b1d) on given sizes, without a parameters file or device.
+struct SetupProbe : Modeling {
+    int wavelet_of(int n, float step, float f, float* out) {
+        nt = n;
+        dt = step;
+        fmax = f;
+        set_wavelet();
+        std::copy(wavelet.begin(), wavelet.end(), out);
+        return tlag;
+    }
+    void b1d_of(int cells, float factor, float* out) {
+        nb = cells;
+        abc_factor = factor;
+        set_b1d();
+        std::copy(b1d.begin(), b1d.end(), out);
+    }
+};
+}  // namespace
 
 extern "C" {
 
-// Modeling::set_wavelet (modeling.cu:36-59); returns tlag.
+// Modeling::set_wavelet (modeling.cu:36-59) on nt samples; returns tlag.
 int fwi3d_host_wavelet(int nt, float dt, float fmax, float* w) {
-    const float t0 = 2.0f * sqrtf((float)M_PI) / fmax;
-    const float fc = fmax / (3.0f * sqrtf((float)M_PI));
-    for (int n = 0; n < nt; n++) {
-        const float td = n * dt - t0;
-        const float arg = (float)(M_PI * M_PI * M_PI * fc * fc * td * td);
-        w[n] = 1e5f * (1.0f - 2.0f * arg) * expf(-arg);
-    }
-    return (int)(t0 / dt) + 1;
+    if (nt < 1 || !w) return -1;
+    return SetupProbe().wavelet_of(nt, dt, fmax, w);
 }
 
-// Modeling::set_abc_dampers 1-D table (modeling.cu:96-99).
+// Modeling::set_b1d, the 1-D Cerjan table of set_abc_dampers (modeling.cu:96-99).
 void fwi3d_host_b1d(int nb, float abc_factor, float* b1d) {
-    for (int i = 0; i < nb; i++) b1d[i] = expf(-powf(abc_factor * (nb - i), 2.0f));
+    if (nb > 0 && b1d) SetupProbe().b1d_of(nb, abc_factor, b1d);
 }
 
 // catch_parameter (admin.cpp:53-88); returns the value length or -1.

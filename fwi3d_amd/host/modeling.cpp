@@ -113,6 +113,11 @@ void Modeling::set_abc_dampers() {
     abc_length = std::stof(catch_parameter("abc_length", parameters));
     abc_factor = std::stof(catch_parameter("abc_factor", parameters));
     nb = (int)(abc_length / dh) + 1;
+    set_b1d();
+}
+
+// modeling.cu:96-99: the 1-D Cerjan table of nb cells.
+void Modeling::set_b1d() {
     b1d.assign(nb, 0.0f);
     for (int i = 0; i < nb; i++) b1d[i] = expf(-powf(abc_factor * (nb - i), 2.0f));
 }

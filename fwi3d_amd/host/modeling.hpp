@@ -69,6 +69,7 @@ protected:
     void set_properties();
     void set_seismograms();
     void set_abc_dampers();
+    void set_b1d();  // b1d from nb and abc_factor
     void set_main_parameters();
     void set_build_parameters();
 

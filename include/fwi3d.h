@@ -105,7 +105,7 @@ int fwi_set_physics(fwi_ctx* ctx, int flags);
  * the FWI gradient term is always added per step, only its illumination
  * deferred.  No reference counterpart: the
  * reference's cross_correlation (migration.cu:181-202) and build_gradient
- * (inversion.cu:491-558) update them in every step. */
+ * (inversion.cu:400-468) update them in every step. */
 int fwi_set_deferred_imaging(fwi_ctx* ctx, int enable);
 
 /* Padded velocity model, host layout (cudaMemcpy of d_Vp, modeling.cu:176).

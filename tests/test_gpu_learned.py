@@ -10,9 +10,9 @@
   the trained network's proposed update points along the true model error
   v_true - v0 (cosine > 0), while the same network before training does not.
   Measured with this set-up: trained cosines 0.26-0.64, untrained -0.21 to
-  -0.15; the bounds below (on the mean over three held-out models) leave
-  margin for the run-to-run variation that FP32 atomic accumulation of the
-  weight gradients gives training.
+  -0.15; the bounds below apply to the mean over three held-out models
+  (training is deterministic: the weight gradients are summed in a fixed
+  order, so a seed fixes the trained network).
 """
 import numpy as np
 import pytest
