@@ -30,8 +30,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-HIP_SOURCES = ["abi.hip", "kernels_step.hip", "kernels_aux.hip", "unet_kernels.hip", "unet_net.hip",
-               "noalloc.hip"]
+HIP_SOURCES = ["abi.hip", "kernels_step.hip", "step_inst_fwd.hip", "step_inst_rtm.hip", "step_inst_fwi.hip",
+               "step_inst_batch.hip", "step_inst_prefetch.hip", "kernels_aux.hip", "unet_kernels.hip", "unet_net.hip", "noalloc.hip"]
 
 CXX = shutil.which("g++") or "g++"
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-ffp-contract=off",

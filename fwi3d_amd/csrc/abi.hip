@@ -688,6 +688,7 @@ int fwi_set_arith(fwi_ctx* c, int arith) {
             c->tune[k].by = 4;
             c->tune[k].rpt = 2;
             c->tune[k].sb = 1;
+            c->tune[k].pd = 1;
         }
     c->arith = arith;
     return FWI_OK;
@@ -1086,9 +1087,27 @@ int fwi_set_tuning(fwi_ctx* c, int kind, int by, int rpt, int zchunks) {
     t.by = by;
     t.rpt = rpt;
     t.zchunks = zchunks;
+    if (!fwi::step_config_supported(kind, c->arith, t)) t.pd = 1;  // shapes without two-plane prefetch
     if (zchunks < 1 || zchunks > (c->g.nzz - 8) || !fwi::step_config_supported(kind, c->arith, t))
         return fail(c, FWI_ERR_ARG, "unsupported tile configuration");
     c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_set_prefetch_depth(fwi_ctx* c, int kind, int pd) {
+    if (!c) return FWI_ERR_ARG;
+    if (kind < 0 || kind > 3 || (pd != 1 && pd != 2)) return fail(c, FWI_ERR_ARG, "kind / prefetch depth");
+    Tuning t = c->tune[kind];
+    t.pd = pd;
+    if (!fwi::step_config_supported(kind, c->arith, t))
+        return fail(c, FWI_ERR_ARG, "no two-plane prefetch instance for this launch shape");
+    c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_get_prefetch_depth(fwi_ctx* c, int kind, int* pd) {
+    if (!c || kind < 0 || kind > 3 || !pd) return FWI_ERR_ARG;
+    *pd = c->tune[kind].pd;
     return FWI_OK;
 }
 
@@ -1111,6 +1130,7 @@ int fwi_set_shots_per_block(fwi_ctx* c, int kind, int sb) {
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
     Tuning t = c->tune[kind];
     t.sb = sb;
+    if (sb != 1) t.pd = 1;  // two-plane prefetch: one shot per block only
     if (sb < 1 || c->nshots % sb != 0) return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
     // keep the current tile if it exists for this sb, else take the first one that does
     if (!fwi::step_config_supported(kind, c->arith, t))
@@ -1137,8 +1157,8 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(sync_geometry(c));
     char key[200];
-    std::snprintf(key, sizeof key, "step kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith, c->g.nxx,
-                  c->g.nyy, c->g.nzz, c->nshots);
+    std::snprintf(key, sizeof key, "step-pd kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith,
+                  c->g.nxx, c->g.nyy, c->g.nzz, c->nshots);
     if (cache_path) {
         std::ifstream in(cache_path);
         std::string line;
@@ -1147,7 +1167,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
             if (bar == std::string::npos || line.substr(0, bar) != key) continue;
             std::istringstream ss(line.substr(bar + 1));
             Tuning t;
-            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb && c->nshots % t.sb == 0 &&
+            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd && c->nshots % t.sb == 0 &&
                 fwi::step_config_supported(kind, c->arith, t)) {
                 c->tune[kind] = t;
                 return FWI_OK;
@@ -1173,12 +1193,14 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     for (int sb : sb_choices) {
         if (c->nshots % sb != 0) continue;
         for (auto& tc : kTileChoices) {
+            for (int pd = 1; pd <= 2; ++pd)
             for (int zc : zc_choices) {
                 Tuning t;
                 t.by = tc[0];
                 t.rpt = tc[1];
                 t.zchunks = zc;
                 t.sb = sb;
+                t.pd = pd;
                 if (!fwi::step_config_supported(kind, c->arith, t) || zc > c->g.nzz - 8) continue;
                 c->tune[kind] = t;
                 // trial steps past nt: no injection, no recording, no data reads
@@ -1211,6 +1233,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << key << "|" << best_t.by << " " << best_t.rpt << " " << best_t.zchunks << " " << best_t.sb << " "
+            << best_t.pd << " "
             << best << "\n";
     }
     return FWI_OK;

@@ -63,6 +63,7 @@ struct Tuning {
     int rpt = 2;      // tile rows per thread (register blocking in y)
     int zchunks = 1;  // blocks along z per (x, y) tile column
     int sb = 1;       // shots per block (batched launches)
+    int pd = 1;       // prefetch depth: planes of loads in flight (1 or 2)
 };
 
 // Arguments of one step launch.  The fields of shot s live at base + s*vol.

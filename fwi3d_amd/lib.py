@@ -38,6 +38,7 @@ EXPORTS = [
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
     "fwi_copy_bandwidth", "fwi_shot_disable", "fwi_residual_noalloc",
+    "fwi_set_prefetch_depth", "fwi_get_prefetch_depth",
 ]
 
 
@@ -153,6 +154,8 @@ def load():
         "fwi_copy_bandwidth": [P, ctypes.c_long, I, ctypes.POINTER(ctypes.c_double)],
         "fwi_shot_disable": [P, I],
         "fwi_residual_noalloc": [P, P, ctypes.c_long, P, P],
+        "fwi_set_prefetch_depth": [P, I, I],
+        "fwi_get_prefetch_depth": [P, I, ctypes.POINTER(I)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -362,6 +365,14 @@ class Context:
 
     def set_tuning(self, kind, by, rpt, zchunks):
         self._chk(self.L.fwi_set_tuning(self.h, kind, by, rpt, zchunks))
+
+    def set_prefetch_depth(self, kind, pd):
+        self._chk(self.L.fwi_set_prefetch_depth(self.h, kind, pd))
+
+    def prefetch_depth(self, kind):
+        v = ctypes.c_int()
+        self._chk(self.L.fwi_get_prefetch_depth(self.h, kind, ctypes.byref(v)))
+        return v.value
 
     def set_graph(self, enable):
         self._chk(self.L.fwi_set_graph(self.h, 1 if enable else 0))

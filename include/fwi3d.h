@@ -258,6 +258,14 @@ int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);
 int fwi_get_tuning(fwi_ctx* ctx, int kind, int* by, int* rpt, int* zchunks);
 int fwi_set_tuning(fwi_ctx* ctx, int kind, int by, int rpt, int zchunks);
 
+/* Planes of loads in flight in the step kernel (1 or 2; part of the tuning,
+ * chosen by fwi_autotune): with 2 the loads of plane z+2 are issued while
+ * plane z is computed.  Two are available for one shot per block and the
+ * tile shapes whose registers allow it; a backward pass with corrections
+ * runs the one-plane instance of the same shape.  No reference counterpart. */
+int fwi_set_prefetch_depth(fwi_ctx* ctx, int kind, int pd);
+int fwi_get_prefetch_depth(fwi_ctx* ctx, int kind, int* pd);
+
 /* Capture the per-step launches of a full pass in a hipGraph (1) or launch
  * them on the stream (0, default). */
 int fwi_set_graph(fwi_ctx* ctx, int enable);

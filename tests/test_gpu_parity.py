@@ -359,3 +359,60 @@ def test_smallest_grids_forward_and_rtm(fl):
             c.rtm_backward()
             assert np.array_equal(c.get_volume(fl.VOL_IMAGE), want[4]), (nxx, nyy, nzz)
             assert np.array_equal(c.get_volume(fl.VOL_ILLUM), want[5]), (nxx, nyy, nzz)
+
+
+@pytest.mark.parametrize("by", [16, 8])
+@pytest.mark.parametrize("mode", ["forward_rbc", "forward_abc", "rtm", "fwi"])
+def test_two_plane_prefetch_bit_exact(fl, mode, by):
+    """Step kernels with two planes of loads in flight (six-phase loop that
+    leaves after a chunk's last plane) against the oracle bit for bit, for
+    z-chunk lengths of every residue modulo 6 (39 planes in 1, 2, 4, 7 or
+    8 chunks: 39; 20 + 19; 10 x 3 + 9; 6 x 6 + 3; 5 x 7 + 4)."""
+    abc = mode == "forward_abc"
+    case = Case(nz=31, nt=61, amp=0.0 if abc else 0.6, abc_factor=0.05 if abc else None)
+    assert case.nzz - 8 == 39
+    P, Pold, seis = _oracle_forward(case, abc=abc)
+    kind = {"forward_rbc": 0, "forward_abc": 1, "rtm": 2, "fwi": 3}[mode]
+    if kind >= 2:
+        rng = np.random.default_rng(5)
+        data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+        want = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=kind == 3,
+                          tlag=case.tlag)
+    for zc in (1, 2, 4, 7, 8):
+        with case.context(fl.Context, abc=abc) as c:
+            c.set_tuning(kind, by, 1, zc)
+            c.set_prefetch_depth(kind, 2)
+            assert c.prefetch_depth(kind) == 2 and c.tuning(kind) == (by, 1, zc)
+            if kind < 2:
+                c.forward(fl.BOUNDARY_ABC if abc else fl.BOUNDARY_RBC)
+                assert np.array_equal(c.seismogram(), seis), zc
+                assert np.array_equal(c.get_volume(fl.VOL_FIELD), P), zc
+                continue
+            c.set_tuning(0, by, 1, zc)
+            c.set_prefetch_depth(0, 2)
+            c.forward(fl.BOUNDARY_RBC)
+            c.set_backward_data(data)
+            (c.grad_backward if kind == 3 else c.rtm_backward)()
+            got = (c.get_volume(fl.VOL_GRAD if kind == 3 else fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM),
+                   c.get_volume(fl.VOL_FIELD), c.get_volume(fl.VOL_RECV))
+        assert np.abs(want[4]).max() > 0
+        for a, b in zip(got, (want[4], want[5], want[0], want[2])):
+            assert np.array_equal(a, b), zc
+
+
+def test_two_plane_prefetch_with_corrections_uses_one_plane_instances(fl):
+    """A backward pass with corrections at prefetch depth 2 runs the
+    one-plane instance of the shape: the same bits as depth 1."""
+    case = Case(nt=50)
+    outs = []
+    for pd in (1, 2):
+        with case.context(fl.Context) as c:
+            c.set_physics(7)
+            c.set_tuning(2, 16, 1, 2)
+            c.set_prefetch_depth(2, pd)
+            c.forward(fl.BOUNDARY_RBC)
+            c.set_backward_data(c.seismogram())
+            c.rtm_backward()
+            outs.append((c.get_volume(fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)))
+    assert np.abs(outs[0][0]).max() > 0
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
