@@ -390,11 +390,12 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
         c.close()
         return {"shots": S, "profiling_pass_steps": pass_steps}
     sb_rtm = c.shots_per_block(lib.KIND_BWD_RTM)
-    c.set_graph(graph)  # each pass's time loop as one captured hipGraph
+    c.set_graph(graph)  # each pass's time loop as one hipGraph, instantiated once and replayed
     c.forward(lib.BOUNDARY_RBC)
     data = [c.seismogram(slot=s) for s in range(S)]
     for s in range(S):
         c.set_backward_data(data[s], slot=s)
+    c.rtm_backward()  # warm-up (with graphs: the backward pass's instantiation)
     c.zero_volume(lib.VOL_IMAGE)
     c.zero_volume(lib.VOL_ILLUM)
     c.sync()
@@ -406,6 +407,7 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     c.sync()
     t_batch = time.perf_counter() - t0
     img_b = c.get_volume(lib.VOL_IMAGE)
+    graphs = c.graph_instantiations()
     c.close()
     log("batched: %.2f s (forward %.0f ms, backward %.0f ms)" % (t_batch, f_ms, b_ms))
 
@@ -431,7 +433,7 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     rel = float(np.linalg.norm((img_b - img_s).astype(np.float64)) / np.linalg.norm(img_s.astype(np.float64)))
     log("per-shot: %.2f s; image rel-L2 batched vs per-shot %.3g" % (t_seq, rel))
     cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
-    return {"shots": S, "graph": graph, "sb_rtm": sb_rtm, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
+    return {"shots": S, "graph": graph, "graph_instantiations": graphs, "sb_rtm": sb_rtm, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
             "shots_per_hour_per_shot": round(3600.0 * S / t_seq, 1),
             "speedup": round(t_seq / t_batch, 3),
             "gcell_upd_per_s_batched": round(cells * w.steps * 3 * S / t_batch / 1e9, 2),
@@ -467,8 +469,9 @@ def main():
     ap.add_argument("--batch-tuning", default="",
                     help="fixed batched launch shapes instead of autotuning, e.g. 'fwd=8,2,16,1;rtm=4,2,1,4' "
                          "(BY, RPT, z-chunks, shots per block)")
-    ap.add_argument("--batch-graph", type=int, default=0, choices=[0, 1],
-                    help="capture the batched passes' time loops in hipGraphs (1) or launch on the stream (0)")
+    ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
+                    help="run the batched passes' time loops as hipGraphs instantiated once (1) or launch on the "
+                         "stream (0)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the blob-boundary and dataset-tool timings")
     ap.add_argument("--batch-shots", type=int, default=16,
@@ -616,12 +619,22 @@ def main():
         r = experiment.run(args.fwi_scale, 0, log=log)
         fwi_res = {"scale": r["scale"], "grid": r["grid"], "shots": r["shots"], "nt": r["nt"],
                    "time_to_misfit_ratio": r["time_to_misfit_ratio"],
-                   "plain_5_iterations_s": r["plain"]["seconds"], "learned_to_same_misfit_s": r["learned"]["seconds"],
+                   "time_to_misfit_ratio_vs_guarded_plain": r["time_to_misfit_ratio_vs_guarded_plain"],
+                   "stand_in_scale": "the 401x401x201 problem and 200 training models of the configuration are "
+                                     "measured by `python -m fwi3d_amd.experiment --scale full` (minutes of GPU "
+                                     "time); this leg runs the same procedure on the %s grid" % r["scale"],
+                   "plain_5_iterations_s": r["plain"]["seconds"],
+                   "guarded_plain_5_iterations_s": r["guarded_plain"]["seconds"],
+                   "learned_s": r["learned"]["seconds"],
+                   "learned_to_plain_misfit_s": r["learned"]["seconds_to_plain_misfit"],
+                   "learned_to_guarded_plain_misfit_s": r["learned"]["seconds_to_guarded_plain_misfit"],
                    "plain_misfit": [round(v, 2) for v in r["plain"]["misfit"]],
+                   "guarded_plain_misfit": [round(v, 2) for v in r["guarded_plain"]["misfit"]],
                    "learned_misfit": [round(v, 2) for v in r["learned"]["misfit"]],
                    "learned_updates": r["learned"]["updates"],
                    "model_rel_l2": {"initial": round(r["plain"]["initial_model_rel_l2"], 4),
                                     "plain": round(r["plain"]["model_rel_l2"], 4),
+                                    "guarded_plain": round(r["guarded_plain"]["model_rel_l2"], 4),
                                     "learned": round(r["learned"]["model_rel_l2"], 4)},
                    "training": {"triples": r["training_triples"], "triples_s": r["triples_seconds"],
                                 "train_s": r["train_seconds"], "loss_first_last": r["train_loss_first_last"]},

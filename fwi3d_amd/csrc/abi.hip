@@ -39,10 +39,13 @@ struct fwi_ctx {
     float idh2 = 0, idh3 = 0, dt2 = 0;
 
     float* vp = nullptr;                 // nshots volumes
-    float* f[2] = {nullptr, nullptr};    // f[fcur] = d_P (current), f[1-fcur] = d_Pold
-    int fcur = 0;
-    float* r[2] = {nullptr, nullptr};    // r[rcur] = d_Pr, r[1-rcur] = d_Prold
-    int rcur = 0;
+    // source-side fields: f[fcur] = d_P (current), f[fold] = d_Pold; the
+    // two spare buffers receive the output of two-step launches
+    float* f[4] = {nullptr, nullptr, nullptr, nullptr};
+    int fcur = 0, fold = 1;
+    // receiver-side fields: r[rcur] = d_Pr, r[rold] = d_Prold, likewise
+    float* r[4] = {nullptr, nullptr, nullptr, nullptr};
+    int rcur = 0, rold = 1;
     float* image = nullptr;              // nshots partial volumes each; group 0 = result
     float* illum = nullptr;
     float* grad = nullptr;
@@ -74,6 +77,18 @@ struct fwi_ctx {
 
     Tuning tune[4];
     bool use_graph = false;
+    // Instantiated pass graphs, replayed while nothing their launches
+    // captured has changed: every setter that changes a launch argument
+    // (wavelet, tuning, physics, geometry, buffers) advances `gen`.
+    struct PassGraph {
+        std::string key;
+        uint64_t gen;
+        hipGraphExec_t exec;
+        int fcur_after, fold_after, rcur_after, rold_after;
+    };
+    std::vector<PassGraph> graphs;
+    uint64_t gen = 0;
+    int graph_instantiations = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
@@ -113,6 +128,9 @@ long ninterior(const fwi_grid& g) {
     return (long)(g.nxx - 2 * g.nb) * (g.nyy - 2 * g.nb) * (g.nzz - 2 * g.nb);
 }
 
+// (a buffer allocated here keeps its address until the context is destroyed,
+// and a pass allocates what it uses before it is captured, so allocations do
+// not invalidate captured pass graphs)
 int alloc_zero(fwi_ctx* c, float** p, long nfloats) {
     if (*p) return FWI_OK;
     HIPCHK(c, hipMalloc((void**)p, nfloats * sizeof(float)));
@@ -156,6 +174,7 @@ int check_slot(fwi_ctx* c, int slot) {
 // change; recorded seismograms and uploaded data are cleared.
 int sync_geometry(fwi_ctx* c) {
     if (!c->geom_dirty) return FWI_OK;
+    c->gen++;
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (void* p : {(void*)c->rec_off, (void*)c->rec_damp, (void*)c->seis_tm, (void*)c->data_tm, (void*)c->seis_tmp,
                     (void*)c->cell_off, (void*)c->cell_start, (void*)c->cell_rec})
@@ -220,9 +239,9 @@ float* vol_ptr(fwi_ctx* c, int slot, int which, bool alloc, int* rc) {
     switch (which) {
         case FWI_VOL_MODEL: p = &c->vp; break;
         case FWI_VOL_FIELD: p = &c->f[c->fcur]; break;
-        case FWI_VOL_FIELD_OLD: p = &c->f[1 - c->fcur]; break;
+        case FWI_VOL_FIELD_OLD: p = &c->f[c->fold]; break;
         case FWI_VOL_RECV: p = &c->r[c->rcur]; break;
-        case FWI_VOL_RECV_OLD: p = &c->r[1 - c->rcur]; break;
+        case FWI_VOL_RECV_OLD: p = &c->r[c->rold]; break;
         case FWI_VOL_IMAGE: p = &c->image; per_slot = false; break;
         case FWI_VOL_ILLUM: p = &c->illum; per_slot = false; break;
         case FWI_VOL_GRAD: p = &c->grad; per_slot = false; break;
@@ -302,13 +321,13 @@ int run_forward(fwi_ctx* c, int boundary, int n0, int n1, bool record = true) {
                                          c->st));
     for (int n = n0; n < n1; ++n) {
         a.cur = c->f[c->fcur];
-        a.old = c->f[1 - c->fcur];
+        a.old = c->f[c->fold];
         a.inj = (n + 1 < n1 && n + 1 < c->g.nt) ? c->idh3 * c->wavelet[n + 1] : 0.0f;
         a.seis_row = (record && c->nrec_total > 0 && n > c->g.tlag && n - c->g.tlag < c->g.nt)
                          ? c->seis_tm + (long)(n - c->g.tlag) * c->nrec_total
                          : nullptr;
         CHKRC(launch_step_timed(c, mode, t, a));
-        c->fcur ^= 1;
+        std::swap(c->fcur, c->fold);
     }
     return FWI_OK;
 }
@@ -344,17 +363,17 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
         } else {
             a.img = !defer ? 1 : (i % 2 == 1 ? 2 : (b + 1 < b1 ? 0 : 1));
         }
-        a.cur = c->f[1 - c->fcur];
+        a.cur = c->f[c->fold];
         a.old = c->f[c->fcur];
         a.rcur = c->r[c->rcur];
-        a.rold = c->r[1 - c->rcur];
+        a.rold = c->r[c->rold];
         // the newer source-side field of step b is the forward field of step
         // nsteps - b, which carries that step's source term
         const int ns = nsteps - b;
         a.inj = (c->phys & FWI_PHYS_SUBTRACT_SOURCE) && ns >= 0 && ns < nt ? c->idh3 * c->wavelet[ns] : 0.0f;
         CHKRC(launch_step_timed(c, mode, t, a));
-        c->fcur ^= 1;
-        c->rcur ^= 1;
+        std::swap(c->fcur, c->fold);
+        std::swap(c->rcur, c->rold);
         if (b + 1 < b1 && b + 1 < nt)
             HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
                                          c->data_tm + (long)(nt - 2 - b) * c->nrec_total, c->idh3, c->st));
@@ -366,7 +385,7 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
         const int last = b1 - b0 - 1;
         const int ill_terms = (last % 3 + 1) % 3;
         const int img_term = mode == fwi::BWD_RTM && last % 2 == 0;
-        HIPCHK(c, fwi::launch_flush_imaging(c->f[1 - c->fcur], c->f[c->fcur], c->r[1 - c->rcur], a.image, a.illum,
+        HIPCHK(c, fwi::launch_flush_imaging(c->f[c->fold], c->f[c->fcur], c->r[c->rold], a.image, a.illum,
                                             c->L, c->nshots, t.sb, c->L.vol, ill_terms, img_term, c->st));
     }
     const int groups = c->nshots / t.sb;
@@ -377,39 +396,84 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     return FWI_OK;
 }
 
-// Run `body` directly on the stream or captured in a hipGraph, bracketed by
-// timing events (recorded after graph instantiation).
+void drop_graphs(fwi_ctx* c) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// Run `body` directly on the stream or as a hipGraph, bracketed by timing
+// events.  A pass's graph is captured and instantiated once and replayed
+// while its key (the pass, the field parity at its start, the setter
+// generation) is unchanged.
 template <typename F>
-int timed_pass(fwi_ctx* c, int launches, F body) {
+int timed_pass(fwi_ctx* c, int launches, F body, const char* pass_key = nullptr) {
     c->kev_used = 0;
     c->kernel_ms = 0.0;
     if (c->use_graph) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        const int saved_f = c->fcur, saved_r = c->rcur;
-        int rc = body();
-        hipError_t e = hipStreamEndCapture(c->st, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            c->fcur = saved_f;
-            c->rcur = saved_r;
-            return rc;
-        }
-        HIPCHK(c, e);
-        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ei != hipSuccess) {
+        // the key names the buffers the pass starts from: forward passes use
+        // the source-side fields only, backward passes both sides
+        char key[160];
+        const bool bwd_pass = pass_key && pass_key[0] == 'b';
+        std::snprintf(key, sizeof key, "%s|f%d%d r%d%d", pass_key ? pass_key : "-", c->fcur, c->fold,
+                      bwd_pass ? c->rcur : 0, bwd_pass ? c->rold : 1);
+        // graphs of an older generation can never match again
+        for (size_t i = 0; i < c->graphs.size();)
+            if (c->graphs[i].gen != c->gen) {
+                (void)hipGraphExecDestroy(c->graphs[i].exec);
+                c->graphs.erase(c->graphs.begin() + i);
+            } else {
+                ++i;
+            }
+        const fwi_ctx::PassGraph* hit = nullptr;
+        if (pass_key)
+            for (const auto& g : c->graphs)
+                if (g.key == key) hit = &g;
+        hipGraphExec_t exec = hit ? hit->exec : nullptr;
+        if (!hit) {
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            const int saved_f = c->fcur, saved_fo = c->fold, saved_r = c->rcur, saved_ro = c->rold;
+            int rc = body();
+            hipError_t e = hipStreamEndCapture(c->st, &graph);
+            if (rc || e != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                c->fcur = saved_f;
+                c->fold = saved_fo;
+                c->rcur = saved_r;
+                c->rold = saved_ro;
+                if (rc) return rc;
+                return fail(c, FWI_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            }
+            hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            c->fcur = saved_f;
-            c->rcur = saved_r;
-            return fail(c, FWI_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            if (ei != hipSuccess) {
+                c->fcur = saved_f;
+                c->fold = saved_fo;
+                c->rcur = saved_r;
+                c->rold = saved_ro;
+                return fail(c, FWI_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            }
+            c->graph_instantiations++;
+            if (pass_key) {
+                if (c->graphs.size() >= 8) {
+                    (void)hipGraphExecDestroy(c->graphs.front().exec);
+                    c->graphs.erase(c->graphs.begin());
+                }
+                c->graphs.push_back({key, c->gen, exec, c->fcur, c->fold, c->rcur, c->rold});
+            }
+        } else {
+            c->fcur = hit->fcur_after;
+            c->fold = hit->fold_after;
+            if (bwd_pass) {  // a forward pass leaves the receiver side as it is
+                c->rcur = hit->rcur_after;
+                c->rold = hit->rold_after;
+            }
         }
         HIPCHK(c, hipEventRecord(c->ev0, c->st));
         HIPCHK(c, hipGraphLaunch(exec, c->st));
         HIPCHK(c, hipEventRecord(c->ev1, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+        if (!pass_key) (void)hipGraphExecDestroy(exec);
     } else {
         HIPCHK(c, hipEventRecord(c->ev0, c->st));
         CHKRC(body());
@@ -608,9 +672,10 @@ int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out
     if (g.nxx < 9 || g.nyy < 9 || g.nzz < 9 || g.nb < 0 || g.nt < 1 || g.tlag < 0 || !(g.dh > 0) || !(g.dt > 0))
         return fail(nullptr, FWI_ERR_ARG, "invalid grid");
     if (nshots < 1) return fail(nullptr, FWI_ERR_ARG, "nshots must be >= 1");
-    // the step kernels address a volume with 32-bit byte offsets below 2^31
-    if (fwi::make_layout(g.nxx, g.nyy, g.nzz).vol * 4 >= (1L << 31))
-        return fail(nullptr, FWI_ERR_RANGE, "padded volume exceeds 2 GiB");
+    // the step kernels address a plane with 32-bit byte offsets below 2^31
+    // (volumes of 2 GiB and more are addressed plane by plane)
+    if (fwi::make_layout(g.nxx, g.nyy, g.nzz).pp * 4 >= (1L << 31))
+        return fail(nullptr, FWI_ERR_RANGE, "padded x-y plane exceeds 2 GiB");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, FWI_ERR_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return fail(nullptr, FWI_ERR_ARG, "device index out of range");
@@ -651,7 +716,15 @@ int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out
     if (hipMemset(c->src_off, 0, nshots * sizeof(long)) != hipSuccess)
         return bail(fail(c, FWI_ERR_HIP, "hipMemset failed"));
     if ((rc = sync_geometry(c))) return bail(rc);
-    for (auto& t : c->tune) t = Tuning{};
+    const bool big = c->L.vol * 4 >= (1L << 31);
+    for (auto& t : c->tune) {
+        t = Tuning{};
+        if (big) {  // the per-plane-addressed instances
+            t.big = 1;
+            t.by = 16;
+            t.rpt = 1;
+        }
+    }
     *out = c;
     return FWI_OK;
 }
@@ -662,7 +735,9 @@ int fwi_destroy(fwi_ctx* c) {
     if (!c) return FWI_OK;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    float* vols[] = {c->vp, c->f[0], c->f[1], c->r[0], c->r[1], c->image, c->illum, c->grad, c->staging,
+    drop_graphs(c);
+    float* vols[] = {c->vp, c->f[0], c->f[1], c->f[2], c->f[3], c->r[0], c->r[1], c->r[2], c->r[3], c->image,
+                     c->illum, c->grad, c->staging,
                      c->b1d, c->rec_damp, c->seis_tm, c->data_tm, c->seis_tmp};
     for (float* p : vols)
         if (p) (void)hipFree(p);
@@ -682,6 +757,7 @@ const char* fwi_last_error(const fwi_ctx* c) { return c ? c->err.c_str() : g_cre
 
 int fwi_set_arith(fwi_ctx* c, int arith) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (arith != FWI_ARITH_REFERENCE && arith != FWI_ARITH_FP32) return fail(c, FWI_ERR_ARG, "arith");
     for (int k = 0; k < 4; ++k)
         if (!fwi::step_config_supported(k, arith, c->tune[k])) {
@@ -802,6 +878,7 @@ int fwi_set_wavelet(fwi_ctx* c, const float* w) {
     for (float x : v)
         if (!std::isfinite(x)) return fail(c, FWI_ERR_NONFINITE, "wavelet holds NaN/Inf");
     c->wavelet.swap(v);
+    c->gen++;  // the source terms are kernel arguments
     return FWI_OK;
 }
 
@@ -869,7 +946,9 @@ int fwi_forward_steps(fwi_ctx* c, int boundary, int n0, int n1) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(check_ready_forward(c, boundary));
     if (n0 < 0 || n1 < n0 || n1 > c->g.nt + c->g.tlag) return fail(c, FWI_ERR_ARG, "step range");
-    int rc = timed_pass(c, n1 - n0, [&] { return run_forward(c, boundary, n0, n1); });
+    char key[64];
+    std::snprintf(key, sizeof key, "fwd b%d %d-%d", boundary, n0, n1);
+    int rc = timed_pass(c, n1 - n0, [&] { return run_forward(c, boundary, n0, n1); }, key);
     c->fwd_done = 0;
     return rc;
 }
@@ -884,8 +963,11 @@ int fwi_forward(fwi_ctx* c, int boundary) {
     if (c->nrec_total > 0)
         HIPCHK(c, hipMemsetAsync(c->seis_tm, 0, (long)c->g.nt * c->nrec_total * sizeof(float), c->st));
     c->fcur = 0;
+    c->fold = 1;
     const int N = c->g.nt + c->g.tlag;
-    int rc = timed_pass(c, N, [&] { return run_forward(c, boundary, 0, N); });
+    char key[64];
+    std::snprintf(key, sizeof key, "fwd b%d 0-%d", boundary, N);
+    int rc = timed_pass(c, N, [&] { return run_forward(c, boundary, 0, N); }, key);
     c->fwd_done = rc == FWI_OK ? 1 : 0;
     return rc;
 }
@@ -950,7 +1032,9 @@ int fwi_backward_steps(fwi_ctx* c, int kind, int b0, int b1) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(sync_geometry(c));
     CHKRC(ensure_backward_buffers(c, kind));
-    return timed_pass(c, b1 - b0, [&] { return run_backward(c, kind, b0, b1); });
+    char key[64];
+    std::snprintf(key, sizeof key, "bwd k%d %d-%d", kind, b0, b1);
+    return timed_pass(c, b1 - b0, [&] { return run_backward(c, kind, b0, b1); }, key);
 }
 
 static int full_backward(fwi_ctx* c, int kind) {
@@ -963,8 +1047,11 @@ static int full_backward(fwi_ctx* c, int kind) {
     HIPCHK(c, hipMemsetAsync(c->r[0], 0, vols * sizeof(float), c->st));
     HIPCHK(c, hipMemsetAsync(c->r[1], 0, vols * sizeof(float), c->st));
     c->rcur = 0;
+    c->rold = 1;
     const int N = c->g.nt + c->g.tlag;
-    int rc = timed_pass(c, N, [&] { return run_backward(c, kind, 0, N); });
+    char key[64];
+    std::snprintf(key, sizeof key, "bwd k%d 0-%d", kind, N);
+    int rc = timed_pass(c, N, [&] { return run_backward(c, kind, 0, N); }, key);
     c->fwd_done = 0;
     return rc;
 }
@@ -1082,6 +1169,7 @@ int fwi_get_tuning(fwi_ctx* c, int kind, int* by, int* rpt, int* zchunks) {
 
 int fwi_set_tuning(fwi_ctx* c, int kind, int by, int rpt, int zchunks) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
     Tuning t = c->tune[kind];
     t.by = by;
@@ -1096,6 +1184,7 @@ int fwi_set_tuning(fwi_ctx* c, int kind, int by, int rpt, int zchunks) {
 
 int fwi_set_prefetch_depth(fwi_ctx* c, int kind, int pd) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (kind < 0 || kind > 3 || (pd != 1 && pd != 2)) return fail(c, FWI_ERR_ARG, "kind / prefetch depth");
     Tuning t = c->tune[kind];
     t.pd = pd;
@@ -1113,6 +1202,7 @@ int fwi_get_prefetch_depth(fwi_ctx* c, int kind, int* pd) {
 
 int fwi_set_physics(fwi_ctx* c, int flags) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (flags & ~(FWI_PHYS_GRAD_DYY | FWI_PHYS_SUBTRACT_SOURCE | FWI_PHYS_ALIGNED_IMAGING))
         return fail(c, FWI_ERR_ARG, "unknown physics flag");
     c->phys = flags;
@@ -1121,12 +1211,14 @@ int fwi_set_physics(fwi_ctx* c, int flags) {
 
 int fwi_set_deferred_imaging(fwi_ctx* c, int enable) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     c->defer_imaging = enable != 0;
     return FWI_OK;
 }
 
 int fwi_set_shots_per_block(fwi_ctx* c, int kind, int sb) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
     Tuning t = c->tune[kind];
     t.sb = sb;
@@ -1152,6 +1244,7 @@ int fwi_get_shots_per_block(fwi_ctx* c, int kind, int* sb) {
 
 int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
     if (kind < 0 || kind > 3 || steps < 1) return fail(c, FWI_ERR_ARG, "kind/steps");
     if (kind == 1 && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
     HIPCHK(c, hipSetDevice(c->device));
@@ -1167,6 +1260,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
             if (bar == std::string::npos || line.substr(0, bar) != key) continue;
             std::istringstream ss(line.substr(bar + 1));
             Tuning t;
+            t.big = c->tune[kind].big;
             if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd && c->nshots % t.sb == 0 &&
                 fwi::step_config_supported(kind, c->arith, t)) {
                 c->tune[kind] = t;
@@ -1201,6 +1295,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
                 t.zchunks = zc;
                 t.sb = sb;
                 t.pd = pd;
+                t.big = saved.big;
                 if (!fwi::step_config_supported(kind, c->arith, t) || zc > c->g.nzz - 8) continue;
                 c->tune[kind] = t;
                 // trial steps past nt: no injection, no recording, no data reads
@@ -1225,7 +1320,8 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     c->ktime = saved_kt;
     // the trial steps ran on the live buffers: clear them
     const long vols = c->L.vol * c->nshots;
-    for (float* p : {c->f[0], c->f[1], c->r[0], c->r[1], c->image, c->illum, c->grad})
+    for (float* p : {c->f[0], c->f[1], c->f[2], c->f[3], c->r[0], c->r[1], c->r[2], c->r[3], c->image, c->illum,
+                     c->grad})
         if (p) HIPCHK(c, hipMemsetAsync(p, 0, vols * sizeof(float), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->fwd_done = 0;
@@ -1242,6 +1338,13 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
 int fwi_set_graph(fwi_ctx* c, int enable) {
     if (!c) return FWI_ERR_ARG;
     c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return FWI_OK;
+}
+
+int fwi_graph_instantiations(fwi_ctx* c, int* count) {
+    if (!c || !count) return FWI_ERR_ARG;
+    *count = c->graph_instantiations;
     return FWI_OK;
 }
 

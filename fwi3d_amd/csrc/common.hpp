@@ -64,6 +64,8 @@ struct Tuning {
     int zchunks = 1;  // blocks along z per (x, y) tile column
     int sb = 1;       // shots per block (batched launches)
     int pd = 1;       // prefetch depth: planes of loads in flight (1 or 2)
+    int big = 0;      // per-plane addressing: set for volumes of 2 GiB and more
+    int tb = 1;       // time steps per launch (2: two-step launches where available)
 };
 
 // Arguments of one step launch.  The fields of shot s live at base + s*vol.
@@ -104,8 +106,47 @@ struct StepArgs {
     const int* rec_start;  // per shot: receivers [rec_start[s], rec_start[s+1])
 };
 
+// Arguments of a two-step launch (step2_kernel.hpp): steps n, n+1 of the
+// random-boundary forward pass or b, b+1 of the RTM backward pass, one shot
+// per block.  Inputs (cur, old) and outputs (out_old = step-1 field,
+// out_cur = step-2 field) are distinct buffers.
+constexpr int kMaxRecPlanes = 4;  // receiver planes per shot a two-step launch looks up
+struct Step2Args {
+    Layout L;
+    long sstride;          // floats between consecutive shots' volumes
+    int nshots;
+    int ntx, nty, zlen, nzc, nitems;
+    const float* vp;
+    const float* cur;      // forward: field of step n; backward: So
+    const float* old;      // forward: field of step n-1; backward: S
+    float* out_old;
+    float* out_cur;
+    const float* rcur;     // backward: R, Ro and their outputs
+    const float* rold;
+    float* rout_old;
+    float* rout_cur;
+    float* image;          // per-shot partials (istride apart)
+    float* illum;
+    long istride;
+    float idh2, dt2, idh3;
+    const long* src_off;   // forward: per shot
+    float inj1, inj2;      // forward: source terms of steps n+1, n+2 (0: none)
+    const long* rec_off;   // forward: gather of step n from cur
+    const int* rec_start;
+    float* seis_row0;      // seismogram row of step n (NULL: none)
+    float* seis_row1;      // seismogram row of step n+1 (NULL: none)
+    const int* rp_z;       // [nshots][kMaxRecPlanes] receiver planes (-1: none)
+    const int* rp_map;     // [nshots][kMaxRecPlanes][ny*px] receiver cell index or -1
+    const int* cell_start; // receivers of cell c: cell_rec[cell_start[c] .. cell_start[c+1])
+    const int* cell_rec;
+    const float* data_row1;  // backward: data injected before step b+1 (NULL: none)
+};
+
 // Host-side launchers (kernels_step.hip).
 hipError_t launch_step(int mode, int arith, const Tuning& t, const StepArgs& a, hipStream_t st);
+// two-step launches (step_inst_two.hip): available for this mode / arithmetic / tile?
+bool step2_supported(int mode, int arith, const Tuning& t);
+hipError_t launch_step2(int mode, int arith, const Tuning& t, const Step2Args& a, hipStream_t st);
 bool step_config_supported(int mode, int arith, const Tuning& t);
 // deferred imaging (StepArgs::img 0/2) available for this backward shape,
 // with or without the corrections of StepArgs::phys

@@ -10,18 +10,20 @@ const Entry* step_table_rtm(int* count);
 const Entry* step_table_fwi(int* count);
 const Entry* step_table_batch(int* count);
 const Entry* step_table_prefetch(int* count);
+const Entry* step_table_big(int* count);
 
 namespace {
 
 const Entry* find(int mode, int arith, const Tuning& t) {
-    int counts[5];
+    int counts[6];
     const Entry* tables[] = {step_table_fwd(&counts[0]), step_table_rtm(&counts[1]), step_table_fwi(&counts[2]),
-                             step_table_batch(&counts[3]), step_table_prefetch(&counts[4])};
-    for (int k = 0; k < 5; ++k)
+                             step_table_batch(&counts[3]), step_table_prefetch(&counts[4]),
+                             step_table_big(&counts[5])};
+    for (int k = 0; k < 6; ++k)
         for (int i = 0; i < counts[k]; ++i) {
             const Entry& e = tables[k][i];
             if (e.mode == mode && e.arith == arith && e.by == t.by && e.rpt == t.rpt && e.sb == t.sb &&
-                e.pd == t.pd)
+                e.pd == t.pd && e.big == t.big)
                 return &e;
         }
     return nullptr;

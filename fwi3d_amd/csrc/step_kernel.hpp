@@ -173,8 +173,8 @@ __device__ __forceinline__ float damper_zon(float bz, float bx, const DampXY& d)
 // OOB, which the hardware range check turns into a load of 0 and a dropped
 // store: the loop body has no per-lane branches, so the compiler can keep
 // several planes of loads and stores in flight with exact wait counts.
-// Volumes are below 2 GiB (checked when the context is created), so an
-// in-range offset plus the plane offset never reaches OOB.
+// For volumes below 2 GiB an in-range offset plus the plane offset never
+// reaches OOB; larger volumes are addressed per plane (Vol<true> below).
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 constexpr unsigned OOB = 0x80000000u;
 constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer resource word 3: 32-bit data, raw addressing
@@ -197,7 +197,49 @@ struct Parity {
     static constexpr int value = P;
 };
 
-template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG, int PD>
+// One volume of a launch, addressed by plane.  Below 2 GiB (BIG = false): a
+// wave-uniform buffer resource over the whole volume, the plane's byte
+// offset as the scalar offset.  From 2 GiB on (BIG = true), where 32-bit
+// byte offsets no longer reach every plane: a resource per access, based at
+// the plane (64-bit base) and covering one plane; planes outside the volume
+// get an empty range.  A plane index outside [0, nz) reads the nearest plane
+// of the volume below 2 GiB and 0 from 2 GiB on (callers discard such
+// values); the lane offset OOB reads 0 and drops stores.
+template <bool BIG>
+struct Vol {
+    Rsrc r;
+    const float* base;
+    long pp;
+    int nz;
+    unsigned plane_b;
+    __device__ __forceinline__ Vol(const float* p, long vol_b, const Layout& L, unsigned pb)
+        : r(make_rsrc(p, BIG ? 0 : vol_b)), base(p), pp(L.pp), nz(L.nz), plane_b(pb) {}
+    __device__ __forceinline__ Vol() : r(), base(nullptr), pp(0), nz(0), plane_b(0) {}
+    __device__ __forceinline__ Rsrc plane(int zz, unsigned* soff) const {
+        if constexpr (!BIG) {
+            *soff = (unsigned)min(max(zz, 0), nz - 1) * plane_b;
+            return r;
+        } else {
+            const bool in = zz >= 0 && zz < nz;
+            *soff = 0;
+            return make_rsrc(base + (in ? (long)zz * pp : 0), in ? (long)plane_b : 0);
+        }
+    }
+    template <int AUX = 0>
+    __device__ __forceinline__ float ld(unsigned voff, int zz) const {
+        unsigned so;
+        const Rsrc rz = plane(zz, &so);
+        return bload<AUX>(rz, voff, so);
+    }
+    template <int AUX = 0>
+    __device__ __forceinline__ void st(unsigned voff, int zz, float v) const {
+        unsigned so;
+        const Rsrc rz = plane(zz, &so);
+        bstore<AUX>(rz, voff, so, v);
+    }
+};
+
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG, int PD, bool BIG>
 __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     constexpr int NF = (MODE == BWD_RTM || MODE == BWD_FWI) ? 2 : 1;
     constexpr bool BWD = NF == 2;
@@ -324,13 +366,13 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
         // this group's shot: buffer resources (wave-uniform)
         const int shot = grp * SB + g;
         const long sbase = (long)shot * a.sstride;
-        Rsrc fb[NF], ob, vb, rb;
-        fb[0] = make_rsrc(a.cur + sbase, vol_b);
-        ob = make_rsrc(a.old + sbase, vol_b);
-        vb = make_rsrc(a.vp + sbase, vol_b);
+        Vol<BIG> fb[NF], ob, vb, rb;
+        fb[0] = Vol<BIG>(a.cur + sbase, vol_b, L, plane_b);
+        ob = Vol<BIG>(a.old + sbase, vol_b, L, plane_b);
+        vb = Vol<BIG>(a.vp + sbase, vol_b, L, plane_b);
         if constexpr (BWD) {
-            fb[1] = make_rsrc(a.rcur + sbase, vol_b);
-            rb = make_rsrc(a.rold + sbase, vol_b);
+            fb[1] = Vol<BIG>(a.rcur + sbase, vol_b, L, plane_b);
+            rb = Vol<BIG>(a.rold + sbase, vol_b, L, plane_b);
         }
         int src_z = -1, src_c = -1;
         if (!BWD || (CORR && (a.phys & PHYS_SUBTRACT_SOURCE))) {  // uniform
@@ -341,8 +383,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             }
         }
         // image/illumination of the shot group: read and written by group 0
-        Rsrc ib = make_rsrc(a.image + (long)grp * a.istride, vol_b);
-        Rsrc lb = make_rsrc(a.illum + (long)grp * a.istride, vol_b);
+        const Vol<BIG> ib(a.image + (long)grp * a.istride, vol_b, L, plane_b);
+        const Vol<BIG> lb(a.illum + (long)grp * a.istride, vol_b, L, plane_b);
         const bool owner = g == 0;
 
         // Pipeline registers.  The loads of plane z + PD (PD = 1 or 2, the
@@ -378,27 +420,26 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             float(&rr)[RPT] = sro[S];
             float(&ii)[RPT] = sim[S];
             float(&ll)[RPT] = sil[S];
-            const unsigned pz = (unsigned)zz * plane_b;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) hx[H][f][r] = bload(fb[f], hxo[r], pz);
+                for (int r = 0; r < RPT; ++r) hx[H][f][r] = fb[f].ld(hxo[r], zz);
 #pragma unroll
                 for (int m = 0; m < HY; ++m)  // wave-uniform: waves holding y-halo rows
-                    if (ty + m * BY < 8) hy[H][f][m] = bload(fb[f], hyo[m], pz);
+                    if (ty + m * BY < 8) hy[H][f][m] = fb[f].ld(hyo[m], zz);
             }
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                oo[r] = bload<kNT>(ob, co[r], pz);
-                vv[r] = bload<kNT>(vb, co[r], pz);
-                if constexpr (BWD) rr[r] = bload<kNT>(rb, co[r], pz);
+                oo[r] = ob.template ld<kNT>(co[r], zz);
+                vv[r] = vb.template ld<kNT>(co[r], zz);
+                if constexpr (BWD) rr[r] = rb.template ld<kNT>(co[r], zz);
             }
             if constexpr (BWD) {
                 if (owner) {
 #pragma unroll
                     for (int r = 0; r < RPT; ++r) {
-                        if constexpr (RMW_IMG) ii[r] = bload<kNT>(ib, co[r], pz);
-                        if constexpr (RMW_ILL) ll[r] = bload<kNT>(lb, co[r], pz);
+                        if constexpr (RMW_IMG) ii[r] = ib.template ld<kNT>(co[r], zz);
+                        if constexpr (RMW_ILL) ll[r] = lb.template ld<kNT>(co[r], zz);
                     }
                 }
             }
@@ -426,7 +467,6 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
         // group 0: add the other groups' terms of plane zz (parity buffer
         // xb), in shot order, and store image and illumination
         auto finish_plane = [&](int zz, int xb) {
-            const unsigned pz = (unsigned)zz * plane_b;
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
                 float ill = ill_r[r], img = img_r[r];
@@ -482,8 +522,8 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
                         if constexpr (RMW_IMG) img = img + xch[xb][2 * k + 1][row][tx];
                     }
                 }
-                if constexpr (RMW_ILL) bstore<kNT>(lb, co[r], pz, ill);
-                if constexpr (RMW_IMG) bstore<kNT>(ib, co[r], pz, img);
+                if constexpr (RMW_ILL) lb.template st<kNT>(co[r], zz, ill);
+                if constexpr (RMW_IMG) ib.template st<kNT>(co[r], zz, img);
             }
         };
 
@@ -493,16 +533,16 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
 #pragma unroll
             for (int d = 0; d < 9; ++d)
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) q[f][r][d] = bload(fb[f], qo[r], (unsigned)(zb - 4 + d) * plane_b);
+                for (int r = 0; r < RPT; ++r) q[f][r][d] = fb[f].ld(qo[r], zb - 4 + d);
         load_plane(zb, Parity<0>{}, Parity<0>{});
         if constexpr (PD == 2) {
             // plane zb+1 in flight: its queue head (zb+5, consumed at the
             // end of plane zb, from head set 1) and its halos and streams
-            const unsigned ph = (unsigned)min(zb + 5, L.nz - 1) * plane_b;
+            const int ph = min(zb + 5, L.nz - 1);
 #pragma unroll
             for (int f = 0; f < NF; ++f)
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) qn[1][f][r] = bload(fb[f], qo[r], ph);
+                for (int r = 0; r < RPT; ++r) qn[1][f][r] = fb[f].ld(qo[r], ph);
             load_plane(zb + 1, Parity<1>{}, Parity<1>{});
         }
         stage_plane(0, 4, Parity<0>{});
@@ -537,11 +577,11 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             // clamped to the last plane of the volume, and planes past it
             // read 0).
             {
-                const unsigned ph = (unsigned)min(z + 4 + PD, L.nz - 1) * plane_b;
+                const int ph = min(z + 4 + PD, L.nz - 1);
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
 #pragma unroll
-                    for (int r = 0; r < RPT; ++r) qn[HL][f][r] = bload(fb[f], qo[r], ph);
+                    for (int r = 0; r < RPT; ++r) qn[HL][f][r] = fb[f].ld(qo[r], ph);
                 load_plane(z + PD, Parity<SN>{}, Parity<HL>{});
             }
 
@@ -721,11 +761,10 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
             // 3. store plane z; stage plane z+1 into the other LDS buffer
             // (the staging waits only for the prefetched loads), advance the queue
             {
-                const unsigned pz = (unsigned)z * plane_b;
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
-                    bstore<kNT>(ob, co[r], pz, res[0][r]);
-                    if constexpr (BWD) bstore<kNT>(rb, co[r], pz, res[1][r]);
+                    ob.template st<kNT>(co[r], z, res[0][r]);
+                    if constexpr (BWD) rb.template st<kNT>(co[r], z, res[1][r]);
                 }
                 if constexpr (BWD && SB == 1) finish_plane(z, 0);
             }
@@ -787,9 +826,9 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     }
 }
 
-template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG, int PD = 1>
+template <int MODE, int ARITH, int BY, int RPT, int SB, bool CORR, int IMG, int PD = 1, bool BIG = false>
 hipError_t launch_one(const StepArgs& a, dim3 grid, hipStream_t st) {
-    step_kernel<MODE, ARITH, BY, RPT, SB, CORR, IMG, PD><<<grid, dim3(64, BY * SB), 0, st>>>(a);
+    step_kernel<MODE, ARITH, BY, RPT, SB, CORR, IMG, PD, BIG><<<grid, dim3(64, BY * SB), 0, st>>>(a);
     return hipGetLastError();
 }
 
@@ -800,6 +839,7 @@ using LaunchFn = hipError_t (*)(const StepArgs&, dim3, hipStream_t);
 struct Entry {
     int mode, arith, by, rpt, sb;
     int pd;            // prefetch depth (planes in flight)
+    int big;           // per-plane addressing (volumes of 2 GiB and more)
     LaunchFn fn;       // per-step imaging, no corrections
     LaunchFn fn_corr;  // with the backward corrections (backward modes)
     LaunchFn fn_img0;  // deferred imaging: none this launch
@@ -833,7 +873,7 @@ constexpr bool cad3_ill(int m, int by, int rpt, int sb) {
 constexpr bool cad3_img(int m, int by, int rpt, int sb) { return m == BWD_RTM && cad3_fits(by, rpt, sb); }
 constexpr bool cad3_img_corr(int m, int sb) { return m == BWD_RTM && defers_corr(m, sb); }
 #define FWI_E(M, A, BY, RPT, SB)                                                                      \
-    {M, A, BY, RPT, SB, 1, &launch_one<M, A, BY, RPT, SB, false, 1>,                                   \
+    {M, A, BY, RPT, SB, 1, 0, &launch_one<M, A, BY, RPT, SB, false, 1>,                                \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, SB, is_bwd(M), 1> : nullptr,                               \
      defers(M, BY, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, BY, SB) ? 0 : 1> : nullptr,   \
      defers(M, BY, SB) ? &launch_one<M, A, BY, RPT, SB, false, defers(M, BY, SB) ? 2 : 1> : nullptr,   \
@@ -861,7 +901,7 @@ constexpr bool cad3_img_corr(int m, int sb) { return m == BWD_RTM && defers_corr
 // imaging instances without the corrections (a launch with corrections
 // takes the one-plane instance of the same shape)
 #define FWI_E2(M, A, BY, RPT)                                                                            \
-    {M, A, BY, RPT, 1, 2, &launch_one<M, A, BY, RPT, 1, false, 1, 2>, nullptr,                            \
+    {M, A, BY, RPT, 1, 2, 0, &launch_one<M, A, BY, RPT, 1, false, 1, 2>, nullptr,                         \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, 1, false, is_bwd(M) ? 0 : 1, 2> : nullptr,                   \
      is_bwd(M) ? &launch_one<M, A, BY, RPT, 1, false, is_bwd(M) ? 2 : 1, 2> : nullptr, nullptr, nullptr,  \
      M == BWD_RTM ? &launch_one<M, A, BY, RPT, 1, false, M == BWD_RTM ? 3 : 1, 2> : nullptr,             \
@@ -870,6 +910,14 @@ constexpr bool cad3_img_corr(int m, int sb) { return m == BWD_RTM && defers_corr
      nullptr, nullptr, nullptr}
 // (shapes whose instances fit 128 registers without spilling)
 #define FWI_TILES2(M, A) FWI_E2(M, A, 16, 1), FWI_E2(M, A, 8, 1)
+
+// volumes of 2 GiB and more: per-plane addressing, one shot per block,
+// image and illumination updated every step (with and without corrections)
+#define FWI_EB(M, A, BY, RPT)                                                                            \
+    {M, A, BY, RPT, 1, 1, 1, &launch_one<M, A, BY, RPT, 1, false, 1, 1, true>,                            \
+     is_bwd(M) ? &launch_one<M, A, BY, RPT, 1, is_bwd(M), 1, 1, true> : nullptr, nullptr, nullptr, nullptr,  \
+     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}
+#define FWI_TILESB(M, A) FWI_EB(M, A, 16, 1), FWI_EB(M, A, 8, 1), FWI_EB(M, A, 4, 2)
 
 // batched: SB wave groups per block, one shot each (BY * SB <= 16 waves)
 #define FWI_BATCH(M)                                                                                     \

@@ -446,4 +446,60 @@ int fwi_unet_conv_probe(fwi_unet* n, int layer, const float* in, int B, int D, i
     return rc;
 }
 
+// Gradient probe of one convolution layer for tests: the data gradient
+// dx = W^T dout ([v][Cin], FP32) and the weight gradient
+// dw = sum_v dout x ([Cout][27][Cin], FP32) of a channels-last input `in`
+// ([v][Cin], rounded to BF16 on upload; with two inputs the first Ca channels
+// form input A and the others input B, as the network's skip concatenation)
+// and an FP32 output gradient dout ([v][Cout]), with the backward kernels of
+// the training step.
+int fwi_unet_conv_grad_probe(fwi_unet* n, int layer, const float* in, const float* dout, int B, int D, int H, int W,
+                             float* dx, float* dw) {
+    if (!n || !in || !dout || !dx || !dw || layer < 0 || layer > 9 || B < 1 || D < 1 || H < 1 || W < 1)
+        return FWI_ERR_ARG;
+    UCHK(n, hipSetDevice(n->device));
+    const ConvSpec& s = kConvs[layer];
+    const int cin = s.ca + s.cb;
+    const long v = (long)B * D * H * W;
+    const long nw = (long)s.cout * 27 * cin;
+    std::vector<bf16> ha(v * s.ca), hb(v * s.cb);
+    for (long q = 0; q < v; ++q) {
+        for (int c = 0; c < s.ca; ++c) ha[q * s.ca + c] = (bf16)in[q * cin + c];
+        for (int c = 0; c < s.cb; ++c) hb[q * s.cb + c] = (bf16)in[q * cin + s.ca + c];
+    }
+    bf16 *da = nullptr, *db = nullptr;
+    float *dgo = nullptr, *ddx = nullptr, *ddw = nullptr, *part = nullptr;
+    const long part_cap = (long)fwi::unet::kWgradChunks * nw;
+    int rc = ualloc(n, &da, v * s.ca);
+    if (!rc && s.cb) rc = ualloc(n, &db, v * s.cb);
+    if (!rc) rc = ualloc(n, &dgo, v * s.cout);
+    if (!rc) rc = ualloc(n, &ddx, v * cin);
+    if (!rc) rc = ualloc(n, &ddw, nw);
+    if (!rc) rc = ualloc(n, &part, part_cap);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(da, ha.data(), v * s.ca * sizeof(bf16), hipMemcpyHostToDevice, n->st);
+        if (e == hipSuccess && s.cb)
+            e = hipMemcpyAsync(db, hb.data(), v * s.cb * sizeof(bf16), hipMemcpyHostToDevice, n->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dgo, dout, v * s.cout * sizeof(float), hipMemcpyHostToDevice, n->st);
+        if (e == hipSuccess) e = hipMemsetAsync(ddx, 0, v * cin * sizeof(float), n->st);
+        if (e == hipSuccess) e = hipMemsetAsync(ddw, 0, nw * sizeof(float), n->st);
+        if (e == hipSuccess)
+            e = fwi::unet::conv_dgrad(dgo, s.cout, n->wd[layer], n->kpadd[layer], cin, ddx, B, D, H, W, n->st);
+        if (e == hipSuccess)
+            e = fwi::unet::conv_wgrad(dgo, s.cout, da, s.ca, db, s.cb, ddw, 27 * cin, B, D, H, W, part, part_cap,
+                                      n->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dx, ddx, v * cin * sizeof(float), hipMemcpyDeviceToHost, n->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dw, ddw, nw * sizeof(float), hipMemcpyDeviceToHost, n->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(n->st);
+        if (e != hipSuccess) rc = ufail(n, FWI_ERR_HIP, hipGetErrorString(e));
+    }
+    ufree(da);
+    ufree(db);
+    ufree(dgo);
+    ufree(ddx);
+    ufree(ddw);
+    ufree(part);
+    return rc;
+}
+
 }  // extern "C"

@@ -132,12 +132,36 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", 
     v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
     out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
                     "initial_model_rel_l2": model_err(v0)}
-    target = hist_plain[-1]
-    v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=target,
-                                                   max_iterations=p.get("max_learned", 10), log=log)
+    # plain FWI with the learned loop's acceptance rule (a step is kept only
+    # when the misfit drops, else halved): the baseline without the network
+    # but with the same guard
+    t_guard = []
+    v_guard, hist_guard, sec_guard, steps_guard = fwi.guarded_fwi(prob, v0, 5, log=log, times=t_guard)
+    out["guarded_plain"] = {"misfit": hist_guard, "seconds": round(sec_guard, 3),
+                            "model_rel_l2": model_err(v_guard), "steps": steps_guard}
+    targets = {"plain": hist_plain[-1], "guarded_plain": hist_guard[-1]}
+    t_learned = []
+    v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=min(targets.values()),
+                                                   max_iterations=p.get("max_learned", 10), log=log,
+                                                   times=t_learned)
     out["learned"] = {"misfit": hist_l, "seconds": round(sec_l, 3), "model_rel_l2": model_err(v_l),
-                      "updates": used, "reached_target": bool(hist_l[-1] <= target)}
-    out["time_to_misfit_ratio"] = round(sec_l / sec_plain, 4) if hist_l[-1] <= target else None
+                      "updates": used, "elapsed_at_misfit": [round(t, 3) for t in t_learned]}
+
+    def time_to(hist, times, target):
+        for f, t in zip(hist, times):
+            if f <= target:
+                return t
+        return None
+
+    ratios = {}
+    for name, target in targets.items():
+        t = time_to(hist_l, t_learned, target)
+        base = sec_plain if name == "plain" else sec_guard
+        ratios[name] = round(t / base, 4) if t is not None else None
+        out["learned"]["seconds_to_%s_misfit" % name] = round(t, 3) if t is not None else None
+    out["learned"]["reached_target"] = ratios["plain"] is not None
+    out["time_to_misfit_ratio"] = ratios["plain"]
+    out["time_to_misfit_ratio_vs_guarded_plain"] = ratios["guarded_plain"]
     prob.close()
     net.close()
     return out

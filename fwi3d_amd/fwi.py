@@ -128,6 +128,47 @@ def gradient_step(prob, v, g, f0, vmin, vmax):
     return trial(step), step
 
 
+def guarded_fwi(prob, v0, iterations, log=None, max_halvings=4, times=None):
+    """Plain FWI with the acceptance rule of the learned loop: a gradient
+    step (parabolic line search, as plain_fwi) is kept only when the misfit
+    of the new model is below the current one; otherwise the step is halved
+    (up to max_halvings times), and the run stops when no halving helps.
+    Returns (v, misfits, seconds, steps_taken); times (a list) receives the
+    elapsed seconds at each misfit."""
+    vmin, vmax = float(v0.min()), float(v0.max())
+    vlo, vhi = velocity_bounds(prob.w)
+    mlo, mhi = np.float32(1.0 / vhi ** 2), np.float32(1.0 / vlo ** 2)
+    v = v0.astype(np.float32)
+    t0 = time.perf_counter()
+    f, g, gmax, illum = prob.gradient(v)
+    hist, taken = [float(np.sqrt(f))], []
+    if times is not None:
+        times.append(time.perf_counter() - t0)
+    for k in range(iterations):
+        m = (1.0 / (v * v)).astype(np.float32)
+        _, step = gradient_step(prob, v, g, hist[-1], vmin, vmax)
+        accepted = False
+        for h in range(max_halvings + 1):
+            v_try = (1.0 / np.sqrt(np.clip(m - np.float32(step) * g, mlo, mhi))).astype(np.float32)
+            f_try, g_try, _, il_try = prob.gradient(v_try)
+            if np.sqrt(f_try) < hist[-1]:
+                v, g, illum = v_try, g_try, il_try
+                hist.append(float(np.sqrt(f_try)))
+                taken.append(float(step))
+                accepted = True
+                break
+            step *= 0.5
+        if times is not None and accepted:
+            times.append(time.perf_counter() - t0)
+        if log:
+            log("guarded FWI iteration %d: misfit %.6g (%s, %.1f s)" % (k + 1, hist[-1],
+                                                                       "accepted" if accepted else "no descent",
+                                                                       time.perf_counter() - t0))
+        if not accepted:
+            break
+    return v, hist, time.perf_counter() - t0, taken
+
+
 def plain_fwi(prob, v0, iterations, record=None, log=None):
     """`iterations` gradient steps from v0; returns (v, misfits, seconds)
     with misfits[k] = sqrt(sum r^2) of the model after k steps.  `record`

@@ -142,11 +142,12 @@ def predict(net, g, illum, v):
     return (y[sl[1:]] * V_SCALE).astype(np.float32)
 
 
-def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
+def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None, times=None):
     """FWI with the learned update and misfit-guarded fallback.  Stops when
     the misfit reaches target_misfit (if given) or after max_iterations.
     Returns (v, misfits, seconds, used) where used[k] says which update was
-    taken ('learned' or 'gradient')."""
+    taken ('learned' or 'gradient'); times (a list) receives the elapsed
+    seconds at each misfit."""
     vmin, vmax = float(v0.min()), float(v0.max())
     vlo, vhi = fwi.velocity_bounds(prob.w)
     v = v0.astype(np.float32)
@@ -154,6 +155,8 @@ def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
     t0 = time.perf_counter()
     f, g, gmax, illum = prob.gradient(v)
     hist.append(float(np.sqrt(f)))
+    if times is not None:
+        times.append(time.perf_counter() - t0)
     for k in range(max_iterations):
         if target_misfit is not None and hist[-1] <= target_misfit:
             break
@@ -176,6 +179,8 @@ def learned_fwi(prob, net, v0, target_misfit=None, max_iterations=5, log=None):
             f, g, gmax, illum = prob.gradient(v)
             hist.append(float(np.sqrt(f)))
             used.append("gradient")
+        if times is not None:
+            times.append(time.perf_counter() - t0)
         if log:
             log("learned FWI iteration %d (%s): misfit %.6g (%.1f s)" % (k + 1, used[-1], hist[-1],
                                                                         time.perf_counter() - t0))

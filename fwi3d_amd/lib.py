@@ -38,7 +38,7 @@ EXPORTS = [
     "fwi_shot_set_source", "fwi_shot_set_receivers", "fwi_shot_get_seismogram", "fwi_shot_set_backward_data",
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
     "fwi_copy_bandwidth", "fwi_shot_disable", "fwi_residual_noalloc",
-    "fwi_set_prefetch_depth", "fwi_get_prefetch_depth",
+    "fwi_set_prefetch_depth", "fwi_get_prefetch_depth", "fwi_graph_instantiations",
 ]
 
 
@@ -156,6 +156,7 @@ def load():
         "fwi_residual_noalloc": [P, P, ctypes.c_long, P, P],
         "fwi_set_prefetch_depth": [P, I, I],
         "fwi_get_prefetch_depth": [P, I, ctypes.POINTER(I)],
+        "fwi_graph_instantiations": [P, ctypes.POINTER(I)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -372,6 +373,11 @@ class Context:
     def prefetch_depth(self, kind):
         v = ctypes.c_int()
         self._chk(self.L.fwi_get_prefetch_depth(self.h, kind, ctypes.byref(v)))
+        return v.value
+
+    def graph_instantiations(self):
+        v = ctypes.c_int()
+        self._chk(self.L.fwi_graph_instantiations(self.h, ctypes.byref(v)))
         return v.value
 
     def set_graph(self, enable):

@@ -15,7 +15,7 @@ _bound = False
 # the fwi_unet_* symbols of include/fwi3d.h
 EXPORTS = ["fwi_unet_create", "fwi_unet_destroy", "fwi_unet_last_error", "fwi_unet_num_params",
            "fwi_unet_get_params", "fwi_unet_set_params", "fwi_unet_forward", "fwi_unet_gradients",
-           "fwi_unet_train_step", "fwi_unet_conv_probe"]
+           "fwi_unet_train_step", "fwi_unet_conv_probe", "fwi_unet_conv_grad_probe"]
 
 
 def _bind():
@@ -37,8 +37,10 @@ def _bind():
     L.fwi_unet_gradients.argtypes = [P, f32, f32, I, I, I, I, ctypes.POINTER(ctypes.c_double), f32]
     L.fwi_unet_train_step.argtypes = [P, f32, f32, I, I, I, I, F, ctypes.POINTER(ctypes.c_double)]
     L.fwi_unet_conv_probe.argtypes = [P, I, f32, I, I, I, I, f32]
+    L.fwi_unet_conv_grad_probe.argtypes = [P, I, f32, f32, I, I, I, I, f32, f32]
     for name in ("fwi_unet_create", "fwi_unet_destroy", "fwi_unet_get_params", "fwi_unet_set_params",
-                 "fwi_unet_forward", "fwi_unet_gradients", "fwi_unet_train_step", "fwi_unet_conv_probe"):
+                 "fwi_unet_forward", "fwi_unet_gradients", "fwi_unet_train_step", "fwi_unet_conv_probe",
+                 "fwi_unet_conv_grad_probe"):
         getattr(L, name).restype = I
     _bound = True
     return L
@@ -123,6 +125,20 @@ class UNet:
         self._chk(self.L.fwi_unet_train_step(self.h, x3, t, B, D, H, W, lr, ctypes.byref(loss)))
         return loss.value
 
+    def conv_grad_probe(self, layer, x, dout):
+        """(dx, dw) of one convolution layer's backward kernels: x
+        [B][D][H][W][Cin] (rounded to BF16), dout [B][D][H][W][Cout] FP32;
+        dw is [Cout][27][Cin]."""
+        x = np.ascontiguousarray(x, np.float32)
+        dout = np.ascontiguousarray(dout, np.float32)
+        B, D, H, W, C = x.shape
+        cout = self.CONV_CHANNELS[layer][1]
+        assert dout.shape == (B, D, H, W, cout)
+        dx = np.zeros(x.shape, np.float32)
+        dw = np.zeros((cout, 27, C), np.float32)
+        self._chk(self.L.fwi_unet_conv_grad_probe(self.h, layer, x, dout, B, D, H, W, dx, dw))
+        return dx, dw
+
     def conv_probe(self, layer, x):
         x = np.ascontiguousarray(x, np.float32)
         B, D, H, W, C = x.shape
@@ -130,3 +146,28 @@ class UNet:
         out = np.zeros((B, D, H, W, cout), np.float32)
         self._chk(self.L.fwi_unet_conv_probe(self.h, layer, x, B, D, H, W, out))
         return out
+
+
+def main(argv=None):
+    """Training steps at a fixed shape on synthetic inputs, for kernel traces:
+    python -m fwi3d_amd.unet --steps 6 --batch 4 --size 32"""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--size", type=int, default=32, help="patch edge (a multiple of 4)")
+    a = ap.parse_args(argv)
+    rng = np.random.default_rng(0)
+    n = a.size
+    x3 = rng.normal(size=(3, a.batch, n, n, n)).astype(np.float32)
+    t = (0.5 * x3[0] - 0.2 * x3[2]).astype(np.float32)
+    with UNet(seed=1) as net:
+        for _ in range(a.steps):
+            loss = net.train_step(x3, t, 1e-3)
+    print("loss after %d steps: %.6f" % (a.steps, loss))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

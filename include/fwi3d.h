@@ -36,7 +36,7 @@ extern "C" {
 #define FWI_ERR_HIP (-2)       /* a HIP runtime call failed */
 #define FWI_ERR_CFL (-3)       /* model violates v*dt/dh <= 0.452856 */
 #define FWI_ERR_STATE (-4)     /* call order: e.g. backward before forward */
-#define FWI_ERR_RANGE (-5)     /* source/receiver outside the updated grid; padded volume >= 2 GiB */
+#define FWI_ERR_RANGE (-5)     /* source/receiver outside the updated grid; padded x-y plane >= 2 GiB */
 #define FWI_ERR_NONFINITE (-6) /* NaN/Inf found in a model or wavefield */
 #define FWI_ERR_NODEV (-7)     /* no HIP device */
 
@@ -75,7 +75,10 @@ typedef struct fwi_ctx fwi_ctx;
 /* Create a context on `device` and allocate the padded model and the two
  * source-side fields.  Stands in for Modeling::set_properties' device
  * allocations (modeling.cu:142-179).  Receiver-side fields, image,
- * illumination and gradient are allocated on first use. */
+ * illumination and gradient are allocated on first use.  Padded volumes of
+ * 2 GiB and more run step-kernel instances that address each z-plane with
+ * its own 64-bit base (one shot per block, image and illumination updated
+ * every step); one padded x-y plane must stay below 2 GiB. */
 int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out);
 int fwi_destroy(fwi_ctx* ctx);
 const char* fwi_last_error(const fwi_ctx* ctx);
@@ -267,8 +270,13 @@ int fwi_set_prefetch_depth(fwi_ctx* ctx, int kind, int pd);
 int fwi_get_prefetch_depth(fwi_ctx* ctx, int kind, int* pd);
 
 /* Capture the per-step launches of a full pass in a hipGraph (1) or launch
- * them on the stream (0, default). */
+ * them on the stream (0, default).  A pass's graph is instantiated once and
+ * replayed by later passes of the same kind and step range, until a setter
+ * changes what its launches captured (wavelet, tuning, physics, imaging
+ * cadence, arithmetic, geometry); fwi_graph_instantiations counts the
+ * instantiations of the context (a diagnostic). */
 int fwi_set_graph(fwi_ctx* ctx, int enable);
+int fwi_graph_instantiations(fwi_ctx* ctx, int* count);
 
 /* Device time of the last full pass's step kernels [ms], from HIP events
  * recorded on the context's stream, and the number of step launches. */
@@ -305,6 +313,14 @@ int fwi_unet_train_step(fwi_unet* net, const float* x3, const float* target, int
 /* One convolution layer alone (FP32 output, no bias/activation), for tests;
  * in is channels-last FP32 [B][D][H][W][Cin] rounded to BF16 on upload. */
 int fwi_unet_conv_probe(fwi_unet* net, int layer, const float* in, int B, int D, int H, int W, float* out);
+/* The backward kernels of one convolution layer alone, for tests: data
+ * gradient dx = W^T dout ([B][D][H][W][Cin]) and weight gradient
+ * dw = sum dout x ([Cout][27][Cin]) of the channels-last input `in`
+ * (rounded to BF16 on upload; skip-concatenated layers take their two inputs
+ * as the first and last channels) and the FP32 output gradient dout
+ * ([B][D][H][W][Cout]). */
+int fwi_unet_conv_grad_probe(fwi_unet* net, int layer, const float* in, const float* dout, int B, int D, int H, int W,
+                             float* dx, float* dw);
 
 /* Device-to-device copy bandwidth [GB/s, read + write] of a plain streaming
  * copy kernel over `bytes`, `reps` times: the measured reference the stencil

@@ -36,12 +36,13 @@ def test_create_without_device_fails_cleanly():
     pytest.skip("a GPU is present")
 
 
-def test_create_rejects_volumes_of_2_gib_and_more():
-    """The step kernels address a padded volume with 32-bit byte offsets below
-    2^31: creation refuses larger grids before touching a device."""
+def test_create_rejects_planes_of_2_gib_and_more():
+    """The step kernels address one padded x-y plane with 32-bit byte
+    offsets below 2^31 (volumes of 2 GiB and more are addressed plane by
+    plane): creation refuses larger planes before touching a device."""
     from fwi3d_amd import lib
     with pytest.raises(lib.FwiError) as e:
-        lib.Context(1024, 1024, 512, 50, 10.0, 1e-3, 10, 2)  # 1024*1024*512*4 B = 2 GiB
+        lib.Context(32768, 16384, 9, 0, 10.0, 1e-3, 10, 2)  # 32768*16384*4 B = 2 GiB per plane
     assert e.value.code == -5  # FWI_ERR_RANGE
     with pytest.raises(lib.FwiError) as e:
         lib.Context(40, 40, 40, 8, 10.0, 1e-3, 10, 2, nshots=0)

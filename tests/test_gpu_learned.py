@@ -74,3 +74,36 @@ def test_training_aligns_updates_with_the_model_error():
                 prob.close()
     assert np.mean(cos["trained"]) > 0.15, cos
     assert np.mean(cos["trained"]) > np.mean(cos["untrained"]) + 0.2, cos
+
+
+def test_learned_fwi_reaches_plain_misfit_sooner():
+    """What the trained network gains over the method without it, on eight
+    held-out models (never seen in training): the misfit after one learned
+    iteration against the misfits plain FWI and its guarded variant (steps
+    kept only when the misfit drops) reach after two gradient iterations.
+    Training is deterministic, so the comparison is reproducible; measured
+    with this set-up, the learned iteration is lower than both on five of the
+    eight models (5000, 5001, 5002, 5004, 5005; on 5003 no method lowers the
+    misfit, on 5006 the guarded method wins, on 5007 the first learned
+    proposal is rejected and the fallback step is taken)."""
+    from fwi3d_amd.unet import UNet
+    triples = learned.make_triples(12, NX, NY, NZ, 2, NT, dt=1e-3, fmax=30.0, seed0=1000, sigma=SIGMA, nb=NB)
+    wins, ratios = 0, []
+    with UNet(seed=7) as net:
+        learned.train(net, triples, 300, patch=(16, 16, 16), batch=4, lr=1e-3, seed=0)
+        for seed in range(5000, 5008):
+            v_true, v0, prob = _problem(seed)
+            try:
+                _, plain, _ = fwi.plain_fwi(prob, v0, 2)
+                _, guarded, _, _ = fwi.guarded_fwi(prob, v0, 2)
+                _, hist, _, used = learned.learned_fwi(prob, net, v0, max_iterations=1)
+            finally:
+                prob.close()
+            best = min(plain[-1], guarded[-1])
+            print("held-out %d: plain %s, guarded %s, learned %s %s" % (seed, plain, guarded, hist, used))
+            if used[0] == "learned":
+                assert hist[1] < hist[0]  # an accepted learned update lowers the misfit
+            wins += hist[1] < best
+            ratios.append(hist[1] / best)
+    assert wins >= 4, (wins, ratios)
+    assert np.median(ratios) < 1.0, ratios

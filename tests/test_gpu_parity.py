@@ -227,6 +227,43 @@ def test_graph_matches_stream(fl):
         assert np.array_equal(c.get_volume(fl.VOL_IMAGE), img_graph)
 
 
+def test_graph_replayed_across_passes(fl):
+    """A pass's hipGraph is instantiated once and replayed: three forward +
+    RTM rounds instantiate two graphs; a new wavelet (a kernel argument)
+    forces a new capture, whose result follows the new wavelet; every
+    replay equals the stream path bit for bit."""
+    case = Case(nt=50)
+    P, Pold, seis = _oracle_forward(case)
+    w2 = (case.wavelet * np.float32(0.5)).astype(np.float32)
+    with case.context(fl.Context) as c:
+        c.set_graph(True)
+        for _ in range(3):
+            c.forward(fl.BOUNDARY_RBC)
+            assert np.array_equal(c.seismogram(), seis)
+            c.set_backward_data(seis)
+            c.zero_volume(fl.VOL_IMAGE)
+            c.zero_volume(fl.VOL_ILLUM)
+            c.rtm_backward()
+            img = c.get_volume(fl.VOL_IMAGE)
+        assert c.graph_instantiations() == 2
+        c.set_wavelet(w2)
+        c.forward(fl.BOUNDARY_RBC)
+        seis2 = c.seismogram()
+        assert c.graph_instantiations() == 3
+        c.set_graph(False)
+        c.forward(fl.BOUNDARY_RBC)
+        assert np.array_equal(c.seismogram(), seis2)
+        c.set_wavelet(case.wavelet)
+        c.forward(fl.BOUNDARY_RBC)
+        c.set_backward_data(seis)
+        c.zero_volume(fl.VOL_IMAGE)
+        c.zero_volume(fl.VOL_ILLUM)
+        c.rtm_backward()
+        assert np.array_equal(c.get_volume(fl.VOL_IMAGE), img)
+    _, _, want2 = O.forward(case.vp, case.nb, case.dh, case.dt, case.nt, case.tlag, w2, case.src, case.rec)
+    assert np.array_equal(seis2, want2)
+
+
 def test_model_setup_matches_oracle(fl):
     case = Case(amp=0.0)
     with case.context(fl.Context) as c:
@@ -416,3 +453,47 @@ def test_two_plane_prefetch_with_corrections_uses_one_plane_instances(fl):
             outs.append((c.get_volume(fl.VOL_IMAGE), c.get_volume(fl.VOL_ILLUM)))
     assert np.abs(outs[0][0]).max() > 0
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_volume_of_2_gib_and_more_bit_exact(fl):
+    """A 1024 x 1024 x 520 padded grid (2.18 GB per volume) runs the
+    per-plane-addressed step instances: forward steps [0, 24) with a seeded
+    random boundary and a receiver patch, then RTM backward steps [0, 8),
+    fields, seismogram rows, image and illumination against the oracle bit
+    for bit (tlag 18: a 200 Hz maximum frequency at 1 ms)."""
+    nxx, nyy, nzz, nb, dh, dt, nt = 1024, 1024, 520, 8, 10.0, 1e-3, 40
+    assert ((nxx + 63) // 64 * 64) * nyy * nzz * 4 >= 2 ** 31
+    wavelet, tlag = O.wavelet(nt, dt, 200.0)
+    z = np.arange(nzz - 2 * nb) * dh
+    col = np.where(z < 2000.0, 1500.0, 2500.0).astype(np.float32)
+    interior = np.ascontiguousarray(np.broadcast_to(col, (nyy - 2 * nb, nxx - 2 * nb, nzz - 2 * nb)))
+    vclamp = np.float32(0.45 * dh / dt)
+    vp = O.random_boundary(O.expand_boundary(interior, nb), nb, 3, 0, 0.6, vclamp)
+    del interior
+    src = (nxx // 2, nyy // 2, nb + 2)
+    xs = np.arange(nxx // 2 - 20, nxx // 2 + 21, 10)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    rec = np.stack([X.ravel(), Y.ravel(), np.full(X.size, nb + 2)], axis=1).astype(np.int32)
+    nf, nbk = 24, 8
+    with fl.Context(nxx, nyy, nzz, nb, dh, dt, nt, tlag) as c:
+        c.set_model(vp)
+        c.set_wavelet(wavelet)
+        c.set_source(*src)
+        c.set_receivers(rec)
+        c.forward_steps(fl.BOUNDARY_RBC, 0, nf)
+        g_P, g_Po, g_seis = c.get_volume(fl.VOL_FIELD), c.get_volume(fl.VOL_FIELD_OLD), c.seismogram()
+        P, Pold, seis = O.forward(vp, nb, dh, dt, nt, tlag, wavelet, src, rec, n_end=nf)
+        assert np.abs(seis[:, :nf - tlag]).max() > 0
+        assert np.array_equal(g_P, P) and np.array_equal(g_Po, Pold)
+        assert np.array_equal(g_seis[:, :nf - tlag], seis[:, :nf - tlag])
+        del g_P, g_Po
+        data = np.random.default_rng(2).standard_normal((len(rec), nt)).astype(np.float32)
+        c.set_backward_data(data)
+        for v in (fl.VOL_RECV, fl.VOL_RECV_OLD, fl.VOL_IMAGE, fl.VOL_ILLUM):
+            c.zero_volume(v)
+        c.backward_steps(0, 0, nbk)
+        got = [c.get_volume(v) for v in (fl.VOL_FIELD, fl.VOL_RECV, fl.VOL_IMAGE, fl.VOL_ILLUM)]
+    S, _, R, _, image, illum = O.backward(vp, dh, dt, nt, data, rec, P, Pold, b_begin=0, b_end=nbk, tlag=tlag)
+    assert np.abs(image).max() > 0
+    for name, a, b in zip(("S", "R", "image", "illum"), got, (S, R, image, illum)):
+        assert np.array_equal(a, b), name
