@@ -463,6 +463,9 @@ def main():
                     help="profiling only: one 1 GiB copy-kernel launch (counter calibration)")
     ap.add_argument("--tuning", default="",
                     help="fixed launch shapes instead of autotuning, e.g. 'fwd=4,2,8;rtm=16,1,1'")
+    ap.add_argument("--two-step", default="",
+                    help="two time steps per launch with fixed shapes, e.g. 'fwd=16,2,4;rtm=16,2,4' (BY, RPT, "
+                         "z-chunks), after the one-step tuning (the autotuner already tries them)")
     ap.add_argument("--fwi-scale", default="small", choices=["none", "tiny", "small", "medium"],
                     help="also run plain FWI against learned-preconditioned FWI (time to equal misfit) at this "
                          "grid scale (fwi3d_amd.experiment)")
@@ -509,6 +512,14 @@ def main():
         ctx.autotune(lib.KIND_FWD_RBC, 4)
         ctx.autotune(lib.KIND_BWD_RTM, 4)
         log("autotune %.1f s: forward %s, rtm %s" % (time.perf_counter() - t0, ctx.tuning(0), ctx.tuning(2)))
+    if args.two_step:
+        kinds = {"fwd": lib.KIND_FWD_RBC, "rtm": lib.KIND_BWD_RTM}
+        for item in args.two_step.split(";"):
+            name, vals = item.split("=")
+            ctx.set_two_step(kinds[name.strip()], True, *[int(v) for v in vals.split(",")])
+    two_step = {k: ctx.two_step(kind) for k, kind in (("forward", lib.KIND_FWD_RBC), ("rtm", lib.KIND_BWD_RTM))}
+    log("two-step launches (active, BY, RPT, z-chunks): %s" % two_step)
+    tuning = {"forward": list(ctx.tuning(lib.KIND_FWD_RBC)), "rtm": list(ctx.tuning(lib.KIND_BWD_RTM))}
     ctx.set_graph(args.graph)
     ctx.set_kernel_timing(not args.graph)
 
@@ -569,12 +580,16 @@ def main():
     if nlaunch_b and nlaunch_f:
         bwd_avg_s = bwd_k / nlaunch_b / 1e3
         fwd_avg_s = fwd_k / nlaunch_f / 1e3
-        bwd_bytes = 2 * BYTES_PER_UPDATE * cells
-        fwd_bytes = BYTES_PER_UPDATE * cells
+        # time steps per launch: 1, or about 2 with two-step launches
+        bwd_spl = N * args.steps / nlaunch_b
+        fwd_spl = N * args.steps / nlaunch_f
+        bwd_bytes = int(round(2 * BYTES_PER_UPDATE * cells * bwd_spl))
+        fwd_bytes = int(round(BYTES_PER_UPDATE * cells * fwd_spl))
         ach = bwd_bytes / bwd_avg_s / 1e9
         by, rpt, _ = ctx.tuning(lib.KIND_BWD_RTM)
         key = "step_kernel<BWD_RTM,%s>_BY=%d_RPT=%d_SB=1" % ("REF" if args.arith == "reference" else "FP32", by, rpt)
-        traffic, traffic_src = measured_traffic(key)
+        # the counter summaries cover the one-step shapes
+        traffic, traffic_src = measured_traffic(key) if two_step["rtm"][0] != 1 else (None, None)
         if traffic:
             extra["traffic_source"] = traffic_src
             extra["traffic_GBs_at_measured_bytes"] = round(traffic / bwd_avg_s / 1e9, 1)
@@ -582,8 +597,9 @@ def main():
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "frac_of_copy_kernel": round(ach / copy_gbs, 4),
                     "kernel": "step_kernel<BWD_RTM> (fused reconstruction + receiver propagation + imaging)",
-                    "avg_launch_us": round(bwd_avg_s * 1e6, 2), "algorithmic_bytes_per_launch": bwd_bytes}
-        extra["forward_kernel"] = {"avg_launch_us": round(fwd_avg_s * 1e6, 2),
+                    "avg_launch_us": round(bwd_avg_s * 1e6, 2), "algorithmic_bytes_per_launch": bwd_bytes,
+                    "steps_per_launch": round(bwd_spl, 4)}
+        extra["forward_kernel"] = {"avg_launch_us": round(fwd_avg_s * 1e6, 2), "steps_per_launch": round(fwd_spl, 4),
                                    "achieved_GBs": round(fwd_bytes / fwd_avg_s / 1e9, 1),
                                    "frac": round(fwd_bytes / fwd_avg_s / 1e9 / HBM_PEAK_GBS, 4)}
     pct_hbm = value * BYTES_PER_UPDATE / HBM_PEAK_GBS
@@ -674,7 +690,9 @@ def main():
         "config": {"workload": "single-shot forward + RTM, 401x401x201 @10 m, nb=50 random boundary, "
                                "nt=3000, dt=0.8 ms, 201x201 receivers",
                    "padded_grid": [w.nxx, w.nyy, w.nzz], "steps_per_pass": N, "arith": args.arith,
-                   "graph": args.graph},
+                   "graph": args.graph,
+                   "tuning": tuning,
+                   "two_step": two_step},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "rtm_shots_per_hour_per_shot_path": round(3600.0 / sec_per_step, 1),

@@ -25,6 +25,7 @@
 
 using fwi::Layout;
 using fwi::StepArgs;
+using fwi::Step2Args;
 using fwi::Tuning;
 
 struct fwi_ctx {
@@ -69,6 +70,13 @@ struct fwi_ctx {
     long* cell_off = nullptr;
     int* cell_start = nullptr;
     int* cell_rec = nullptr;
+    // receiver planes of the two-step launches: rp_z [nshots][kMaxRecPlanes]
+    // (-1: none) and per plane a map [ny*px] of receiver cell index or -1;
+    // rp_ok is false when a slot's receivers lie on more planes than that
+    int* rp_z = nullptr;
+    int* rp_map = nullptr;
+    bool rp_ok = false;
+    std::vector<long> cell_off_h;        // host copy of cell_off
     float* seis_tm = nullptr;            // [t][q] over all slots' receivers
     float* data_tm = nullptr;            // [t][q] backward data
     float* seis_tmp = nullptr;           // 2 * nt * nrec_total scratch for transposes
@@ -170,6 +178,52 @@ int check_slot(fwi_ctx* c, int slot) {
     return FWI_OK;
 }
 
+bool two_step_wanted(const fwi_ctx* c) {
+    for (const Tuning& t : c->tune)
+        if (t.tb == 2) return true;
+    return false;
+}
+
+// Receiver planes of every slot for the two-step launches, which find the
+// receiver cells of a plane through a per-slot map of that plane; coff are
+// the receiver cells (absolute offsets) in cell-index order.
+int build_rec_planes(fwi_ctx* c, const std::vector<long>& coff) {
+    c->rp_ok = false;
+    if (!two_step_wanted(c)) return FWI_OK;
+    const int S = c->nshots, K = fwi::kMaxRecPlanes;
+    const long plane_n = (long)c->L.ny * c->L.px;
+    if ((long)S * K * plane_n * (long)sizeof(int) >= (1L << 31)) return FWI_OK;  // offsets of the map buffer
+    if (!c->rp_z) HIPCHK(c, hipMalloc((void**)&c->rp_z, (size_t)S * K * sizeof(int)));
+    if (!c->rp_map) HIPCHK(c, hipMalloc((void**)&c->rp_map, (size_t)S * K * plane_n * sizeof(int)));
+    std::vector<int> z((size_t)S * K, -1);
+    std::vector<long> pos(coff.size());
+    for (size_t j = 0; j < coff.size(); ++j) {
+        const int s = (int)(coff[j] / c->L.vol);
+        const long w = coff[j] - (long)s * c->L.vol;
+        const int zz = (int)(w / c->L.pp);
+        int k = 0;
+        while (k < K && z[s * K + k] != -1 && z[s * K + k] != zz) ++k;
+        if (k == K) return FWI_OK;  // more receiver planes than a launch looks up
+        z[s * K + k] = zz;
+        pos[j] = ((long)s * K + k) * plane_n + (w - (long)zz * c->L.pp);
+    }
+    HIPCHK(c, hipMemcpy(c->rp_z, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
+    long* d_pos = nullptr;
+    if (!pos.empty()) {
+        HIPCHK(c, hipMalloc((void**)&d_pos, pos.size() * sizeof(long)));
+        hipError_t e = hipMemcpy(d_pos, pos.data(), pos.size() * sizeof(long), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = fwi::launch_cell_map(c->rp_map, (long)S * K * plane_n, d_pos, (int)pos.size(), c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        (void)hipFree(d_pos);
+        if (e != hipSuccess) return fail(c, FWI_ERR_HIP, std::string("receiver-plane maps: ") + hipGetErrorString(e));
+    } else {
+        HIPCHK(c, fwi::launch_cell_map(c->rp_map, (long)S * K * plane_n, nullptr, 0, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    c->rp_ok = true;
+    return FWI_OK;
+}
+
 // (Re)build the flattened receiver arrays of all slots after a geometry
 // change; recorded seismograms and uploaded data are cleared.
 int sync_geometry(fwi_ctx* c) {
@@ -191,7 +245,8 @@ int sync_geometry(fwi_ctx* c) {
     const int n = c->nrec_total = c->rec_start_h[S];
     HIPCHK(c, hipMemcpy(c->rec_start, c->rec_start_h.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
     c->geom_dirty = false;
-    if (n == 0) return FWI_OK;
+    c->cell_off_h.clear();
+    if (n == 0) return build_rec_planes(c, c->cell_off_h);
     std::vector<long> off(n);
     std::vector<float> damp(n, 1.0f);
     for (int s = 0; s < S; ++s)
@@ -229,7 +284,8 @@ int sync_geometry(fwi_ctx* c) {
     HIPCHK(c, hipMemcpy(c->cell_rec, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(c->seis_tm, 0, ns * sizeof(float)));
     HIPCHK(c, hipMemset(c->data_tm, 0, ns * sizeof(float)));
-    return FWI_OK;
+    c->cell_off_h = coff;
+    return build_rec_planes(c, coff);
 }
 
 float* vol_ptr(fwi_ctx* c, int slot, int which, bool alloc, int* rc) {
@@ -260,6 +316,8 @@ int ensure_backward_buffers(fwi_ctx* c, int kind) {
     CHKRC(alloc_slots(c, &c->r[1]));
     CHKRC(alloc_slots(c, &c->illum));
     CHKRC(alloc_slots(c, kind == 0 ? &c->image : &c->grad));
+    if (kind == 0 && c->tune[fwi::BWD_RTM].tb == 2)  // the outputs of two-step launches
+        for (float** p : {&c->f[2], &c->f[3], &c->r[2], &c->r[3]}) CHKRC(alloc_slots(c, p));
     return FWI_OK;
 }
 
@@ -307,6 +365,77 @@ StepArgs base_args(fwi_ctx* c) {
     return a;
 }
 
+// The one-step tuning's two-step counterpart (tile and z chunks of the
+// two-step launches).
+Tuning two_step_tuning(const Tuning& t) {
+    Tuning u = t;
+    u.by = t.tby;
+    u.rpt = t.trpt;
+    u.zchunks = t.tzc;
+    return u;
+}
+
+// Two-step launches for this pass: requested, available for the mode,
+// arithmetic and tile, one shot per block, no per-plane addressing, the
+// receiver planes mapped, and (backward) no corrections.
+bool two_step_eligible(const fwi_ctx* c, int mode) {
+    const Tuning& t = c->tune[mode];
+    if (t.tb != 2 || !c->rp_ok || t.sb != 1 || t.big) return false;
+    if (mode == fwi::BWD_RTM && c->phys != 0) return false;
+    if (mode != fwi::FWD_RBC && mode != fwi::BWD_RTM) return false;
+    return fwi::step2_supported(mode, c->arith, two_step_tuning(t));
+}
+
+// ... and the spare buffers it writes are allocated
+bool two_step_active(const fwi_ctx* c, int mode) {
+    if (!c->f[2] || !c->f[3] || (mode == fwi::BWD_RTM && (!c->r[2] || !c->r[3]))) return false;
+    return two_step_eligible(c, mode);
+}
+
+// The two buffer indices (of four) not in {a, b}.
+void spare_pair(int a, int b, int* x, int* y) {
+    int k = 0, out[2] = {0, 0};
+    for (int i = 0; i < 4 && k < 2; ++i)
+        if (i != a && i != b) out[k++] = i;
+    *x = out[0];
+    *y = out[1];
+}
+
+Step2Args base_args2(fwi_ctx* c) {
+    Step2Args a{};
+    a.L = c->L;
+    a.sstride = c->L.vol;
+    a.nshots = c->nshots;
+    a.vp = c->vp;
+    a.istride = c->L.vol;
+    a.idh2 = c->idh2;
+    a.dt2 = c->dt2;
+    a.idh3 = c->idh3;
+    a.src_off = c->src_off;
+    a.rec_off = c->rec_off;
+    a.rec_start = c->rec_start;
+    a.rp_z = c->rp_z;
+    a.rp_map = c->rp_map;
+    a.cell_start = c->cell_start;
+    a.cell_rec = c->cell_rec;
+    return a;
+}
+
+int launch_step2_timed(fwi_ctx* c, int mode, const Tuning& t, const Step2Args& a) {
+    if (!c->ktime || c->use_graph) {
+        HIPCHK(c, fwi::launch_step2(mode, c->arith, t, a, c->st));
+        return FWI_OK;
+    }
+    hipEvent_t e0, e1;
+    CHKRC(kernel_event(c, 2 * c->kev_used, &e0));
+    CHKRC(kernel_event(c, 2 * c->kev_used + 1, &e1));
+    HIPCHK(c, hipEventRecord(e0, c->st));
+    HIPCHK(c, fwi::launch_step2(mode, c->arith, t, a, c->st));
+    HIPCHK(c, hipEventRecord(e1, c->st));
+    c->kev_used++;
+    return FWI_OK;
+}
+
 // Forward steps [n0, n1) of every slot from the current state: the current
 // field excludes the source term of step n0, which is added first; each step
 // kernel adds the source term of the next step to the field it writes
@@ -319,13 +448,39 @@ int run_forward(fwi_ctx* c, int boundary, int n0, int n1, bool record = true) {
     if (n0 < n1 && n0 < c->g.nt)
         HIPCHK(c, fwi::launch_add_points(c->f[c->fcur], c->src_off, c->nshots, c->L.vol, c->idh3 * c->wavelet[n0],
                                          c->st));
-    for (int n = n0; n < n1; ++n) {
+    auto seis_row = [&](int n) -> float* {
+        return (record && c->nrec_total > 0 && n > c->g.tlag && n - c->g.tlag < c->g.nt)
+                   ? c->seis_tm + (long)(n - c->g.tlag) * c->nrec_total
+                   : nullptr;
+    };
+    auto inj = [&](int n) { return (n < n1 && n < c->g.nt) ? c->idh3 * c->wavelet[n] : 0.0f; };
+    int n = n0;
+    if (two_step_active(c, mode)) {
+        // steps n, n+1 per launch: the step-1 field goes to one spare buffer
+        // (the new old field), the step-2 field to the other (the new current)
+        const Tuning t2 = two_step_tuning(t);
+        Step2Args a2 = base_args2(c);
+        for (; n + 1 < n1; n += 2) {
+            int w1, w2;
+            spare_pair(c->fcur, c->fold, &w1, &w2);
+            a2.cur = c->f[c->fcur];
+            a2.old = c->f[c->fold];
+            a2.out_old = c->f[w1];
+            a2.out_cur = c->f[w2];
+            a2.inj1 = inj(n + 1);
+            a2.inj2 = inj(n + 2);
+            a2.seis_row0 = seis_row(n);
+            a2.seis_row1 = seis_row(n + 1);
+            CHKRC(launch_step2_timed(c, mode, t2, a2));
+            c->fold = w1;
+            c->fcur = w2;
+        }
+    }
+    for (; n < n1; ++n) {
         a.cur = c->f[c->fcur];
         a.old = c->f[c->fold];
-        a.inj = (n + 1 < n1 && n + 1 < c->g.nt) ? c->idh3 * c->wavelet[n + 1] : 0.0f;
-        a.seis_row = (record && c->nrec_total > 0 && n > c->g.tlag && n - c->g.tlag < c->g.nt)
-                         ? c->seis_tm + (long)(n - c->g.tlag) * c->nrec_total
-                         : nullptr;
+        a.inj = inj(n + 1);
+        a.seis_row = seis_row(n);
         CHKRC(launch_step_timed(c, mode, t, a));
         std::swap(c->fcur, c->fold);
     }
@@ -352,9 +507,47 @@ int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     // other launch and the illumination every third, and the terms still
     // pending at the end are added by one elementwise launch.  Either way no
     // term is left over when the pass ends.
-    const bool defer = c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
+    // two-step launches add both steps' image and illumination terms
+    // themselves; a segment of odd length ends with a one-step launch that
+    // does the same
+    const bool two = two_step_active(c, mode);
+    const bool defer = !two && c->defer_imaging && fwi::step_defer_supported(mode, c->arith, t, c->phys != 0);
     const bool defer3 = defer && fwi::step_defer3_supported(mode, c->arith, t, c->phys);
-    for (int b = b0; b < b1; ++b) {
+    int b = b0;
+    if (two) {
+        // steps b, b+1 per launch; the data of step b+1 is injected into the
+        // step-1 receiver field inside the launch
+        const Tuning t2 = two_step_tuning(t);
+        Step2Args a2 = base_args2(c);
+        a2.image = c->image;
+        a2.illum = c->illum;
+        for (; b + 1 < b1; b += 2) {
+            int w1, w2, v1, v2;
+            spare_pair(c->fcur, c->fold, &w1, &w2);
+            spare_pair(c->rcur, c->rold, &v1, &v2);
+            a2.cur = c->f[c->fold];
+            a2.old = c->f[c->fcur];
+            a2.out_old = c->f[w1];
+            a2.out_cur = c->f[w2];
+            a2.rcur = c->r[c->rcur];
+            a2.rold = c->r[c->rold];
+            a2.rout_old = c->r[v1];
+            a2.rout_cur = c->r[v2];
+            a2.data_row1 =
+                (b + 1 < nt && c->nrec_total > 0) ? c->data_tm + (long)(nt - 2 - b) * c->nrec_total : nullptr;
+            CHKRC(launch_step2_timed(c, mode, t2, a2));
+            // as after two one-step launches: fold holds the newest
+            // source-side field, rcur the newest receiver field
+            c->fcur = w1;
+            c->fold = w2;
+            c->rold = v1;
+            c->rcur = v2;
+            if (b + 2 < b1 && b + 2 < nt)
+                HIPCHK(c, fwi::launch_inject(c->r[c->rcur], c->cell_off, c->cell_start, c->cell_rec, c->ncells,
+                                             c->data_tm + (long)(nt - 3 - b) * c->nrec_total, c->idh3, c->st));
+        }
+    }
+    for (; b < b1; ++b) {
         const int i = b - b0;
         if (defer3) {
             const bool img_now = mode == fwi::BWD_RTM && i % 2 == 1;
@@ -499,6 +692,8 @@ int check_ready_forward(fwi_ctx* c, int boundary) {
     for (int s = 0; s < c->nshots; ++s)  // -2: a disabled slot (fwi_shot_disable)
         if (c->sx[s] == -1) return fail(c, FWI_ERR_STATE, "source not set for every shot slot");
     if (boundary == FWI_BOUNDARY_ABC && !c->b1d) return fail(c, FWI_ERR_STATE, "dampers not set");
+    if (boundary == FWI_BOUNDARY_RBC && c->tune[fwi::FWD_RBC].tb == 2)  // the outputs of two-step launches
+        for (float** p : {&c->f[2], &c->f[3]}) CHKRC(alloc_slots(c, p));
     return sync_geometry(c);
 }
 
@@ -742,7 +937,7 @@ int fwi_destroy(fwi_ctx* c) {
     for (float* p : vols)
         if (p) (void)hipFree(p);
     void* others[] = {c->src_off, c->rec_off, c->rec_start, c->cell_off, c->cell_start, c->cell_rec,
-                      c->red_partial, c->red_result};
+                      c->red_partial, c->red_result, c->rp_z, c->rp_map};
     for (void* p : others)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
@@ -1200,6 +1395,45 @@ int fwi_get_prefetch_depth(fwi_ctx* c, int kind, int* pd) {
     return FWI_OK;
 }
 
+int fwi_set_two_step(fwi_ctx* c, int kind, int enable, int by, int rpt, int zchunks) {
+    if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
+    if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
+    Tuning t = c->tune[kind];
+    t.tb = enable ? 2 : 1;
+    if (enable) {
+        t.tby = by;
+        t.trpt = rpt;
+        t.tzc = zchunks;
+        Tuning u = t;
+        u.by = by;
+        u.rpt = rpt;
+        u.zchunks = zchunks;
+        u.sb = 1;
+        u.big = 0;
+        if ((kind != fwi::FWD_RBC && kind != fwi::BWD_RTM) || !fwi::step2_supported(kind, c->arith, u))
+            return fail(c, FWI_ERR_ARG, "no two-step instance for this pass kind / tile");
+        if (zchunks < 1 || zchunks > c->g.nzz - 8) return fail(c, FWI_ERR_ARG, "z chunks");
+    }
+    c->tune[kind] = t;
+    // the receiver-plane maps of the current geometry
+    if (enable && !c->geom_dirty && !c->rp_ok) {
+        HIPCHK(c, hipSetDevice(c->device));
+        CHKRC(build_rec_planes(c, c->cell_off_h));
+    }
+    return FWI_OK;
+}
+
+int fwi_get_two_step(fwi_ctx* c, int kind, int* active, int* by, int* rpt, int* zchunks) {
+    if (!c || kind < 0 || kind > 3 || !active || !by || !rpt || !zchunks) return FWI_ERR_ARG;
+    const Tuning& t = c->tune[kind];
+    *active = two_step_eligible(c, kind) ? 1 : (t.tb == 2 ? -1 : 0);
+    *by = t.tby;
+    *rpt = t.trpt;
+    *zchunks = t.tzc;
+    return FWI_OK;
+}
+
 int fwi_set_physics(fwi_ctx* c, int flags) {
     if (!c) return FWI_ERR_ARG;
     c->gen++;  // launch arguments may change: recapture pass graphs
@@ -1250,7 +1484,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(sync_geometry(c));
     char key[200];
-    std::snprintf(key, sizeof key, "step-pd kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith,
+    std::snprintf(key, sizeof key, "step-tb kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith,
                   c->g.nxx, c->g.nyy, c->g.nzz, c->nshots);
     if (cache_path) {
         std::ifstream in(cache_path);
@@ -1261,9 +1495,11 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
             std::istringstream ss(line.substr(bar + 1));
             Tuning t;
             t.big = c->tune[kind].big;
-            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd && c->nshots % t.sb == 0 &&
-                fwi::step_config_supported(kind, c->arith, t)) {
+            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd >> t.tb >> t.tby >> t.trpt >> t.tzc &&
+                c->nshots % t.sb == 0 && fwi::step_config_supported(kind, c->arith, t) &&
+                (t.tb == 1 || fwi::step2_supported(kind, c->arith, two_step_tuning(t)))) {
                 c->tune[kind] = t;
+                if (t.tb == 2 && !c->geom_dirty) CHKRC(build_rec_planes(c, c->cell_off_h));
                 return FWI_OK;
             }
         }
@@ -1275,6 +1511,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     // update every second or third, and another count would add updates at
     // the end that whole passes have only once
     if (bwd) steps = (steps + 5) / 6 * 6;
+    steps += steps % 2;  // whole two-step launches
     const Tuning saved = c->tune[kind];
     const bool saved_graph = c->use_graph, saved_kt = c->ktime;
     c->use_graph = false;
@@ -1296,6 +1533,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
                 t.sb = sb;
                 t.pd = pd;
                 t.big = saved.big;
+                t.tb = 1;
                 if (!fwi::step_config_supported(kind, c->arith, t) || zc > c->g.nzz - 8) continue;
                 c->tune[kind] = t;
                 // trial steps past nt: no injection, no recording, no data reads
@@ -1315,6 +1553,42 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
         }
         if (rc) break;
     }
+    // two-step launches (random-boundary forward, RTM backward): tiles and z
+    // chunks, kept when faster than the best one-step configuration; the
+    // leftover step of an odd segment runs that configuration
+    if (!rc && (kind == fwi::FWD_RBC || kind == fwi::BWD_RTM) && best_t.sb == 1 && !best_t.big &&
+        !(kind == fwi::BWD_RTM && c->phys)) {
+        for (float** p : {&c->f[2], &c->f[3], &c->r[2], &c->r[3]})
+            if (kind == fwi::BWD_RTM || p == &c->f[2] || p == &c->f[3])
+                if ((rc = alloc_slots(c, p))) break;
+        Tuning base = best_t;
+        base.tb = 2;
+        c->tune[kind] = base;
+        if (!rc) rc = build_rec_planes(c, c->cell_off_h);
+        const int tiles2[][2] = {{16, 2}, {12, 3}};
+        const int zc2[] = {1, 2, 3, 4, 5, 6, 8};
+        for (auto& tc : tiles2) {
+            if (rc || !c->rp_ok) break;
+            for (int zc : zc2) {
+                Tuning t = base;
+                t.tby = tc[0];
+                t.trpt = tc[1];
+                t.tzc = zc;
+                if (!fwi::step2_supported(kind, c->arith, two_step_tuning(t)) || zc > c->g.nzz - 8) continue;
+                c->tune[kind] = t;
+                auto body = [&](int nsteps) {
+                    if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
+                    return run_forward(c, FWI_BOUNDARY_RBC, c->g.nt, c->g.nt + nsteps, false);
+                };
+                if ((rc = timed_pass(c, 1, [&] { return body(2); }))) break;
+                if ((rc = timed_pass(c, steps, [&] { return body(steps); }))) break;
+                if (c->last_ms < best) {
+                    best = c->last_ms;
+                    best_t = t;
+                }
+            }
+        }
+    }
     c->tune[kind] = rc ? saved : best_t;
     c->use_graph = saved_graph;
     c->ktime = saved_kt;
@@ -1329,7 +1603,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << key << "|" << best_t.by << " " << best_t.rpt << " " << best_t.zchunks << " " << best_t.sb << " "
-            << best_t.pd << " "
+            << best_t.pd << " " << best_t.tb << " " << best_t.tby << " " << best_t.trpt << " " << best_t.tzc << " "
             << best << "\n";
     }
     return FWI_OK;

@@ -66,6 +66,8 @@ struct Tuning {
     int pd = 1;       // prefetch depth: planes of loads in flight (1 or 2)
     int big = 0;      // per-plane addressing: set for volumes of 2 GiB and more
     int tb = 1;       // time steps per launch (2: two-step launches where available)
+    // tile and z chunks of the two-step launches (tb = 2)
+    int tby = 16, trpt = 2, tzc = 1;
 };
 
 // Arguments of one step launch.  The fields of shot s live at base + s*vol.
@@ -173,6 +175,8 @@ hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t s
 hipError_t launch_flush_imaging(const float* cur, const float* old, const float* rold, float* image, float* illum,
                                 const Layout& L, int nshots, int sb, long istride, int ill_terms, int img_term,
                                 hipStream_t st);
+// every entry of map[map_n] -1, then map[pos[j]] = j for j < ncells
+hipError_t launch_cell_map(int* map, long map_n, const long* pos, int ncells, hipStream_t st);
 hipError_t launch_inject(float* field, const long* cell_off, const int* cell_start, const int* cell_rec,
                          int ncells, const float* data_row, float idh3, hipStream_t st);
 hipError_t launch_export_image(const float* image, const float* illum, float* out, const Layout& L, int nb,

@@ -352,6 +352,18 @@ __global__ void __launch_bounds__(256) inject_kernel(float* __restrict__ field, 
     field[off] = v;
 }
 
+// Receiver-plane maps of the two-step launches: every entry -1, then entry
+// pos[j] = j for each receiver cell j.
+__global__ void __launch_bounds__(256) fill_i32_kernel(int* __restrict__ v, long n, int value) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) v[q] = value;
+}
+
+__global__ void __launch_bounds__(256) cell_map_kernel(int* __restrict__ map, const long* __restrict__ pos,
+                                                       int ncells) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ncells) map[pos[j]] = j;
+}
+
 // Migration::export_seismic (migration.cu:90-117) into the reference
 // interior layout: I = image/illum, out = -(I[i-1] - 2I[i] + I[i+1])*idh2.
 __global__ void __launch_bounds__(256) export_image_kernel(const float* __restrict__ image,
@@ -577,6 +589,12 @@ hipError_t launch_flush_imaging(const float* cur, const float* old, const float*
     const dim3 grid((unsigned)std::min((n + 255) / 256, 64), (unsigned)std::min(L.nz - 8, 4096),
                     (unsigned)(nshots / sb));
     flush_imaging_kernel<<<grid, 256, 0, st>>>(cur, old, rold, image, illum, L, sb, istride, ill_terms, img_term);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_map(int* map, long map_n, const long* pos, int ncells, hipStream_t st) {
+    fill_i32_kernel<<<2048, 256, 0, st>>>(map, map_n, -1);
+    if (ncells > 0) cell_map_kernel<<<(ncells + 255) / 256, 256, 0, st>>>(map, pos, ncells);
     return hipGetLastError();
 }
 

@@ -56,7 +56,9 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
     constexpr int LH = TY + 8;
     constexpr int HY = (8 + BY - 1) / BY;
     __shared__ float lds1[2][NF][LH][LW];
-    __shared__ float lds2[2][NF][TY][64];
+    // same shape as lds1: the 4-cell margin is never written; lanes of the
+    // region's rim read it and discard their step-2 results
+    __shared__ float lds2[2][NF][LH][LW];
 
     const Layout& L = a.L;
     const int tx = threadIdx.x;
@@ -169,6 +171,9 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
         int rp_z[kMaxRecPlanes];
 #pragma unroll
         for (int k = 0; k < kMaxRecPlanes; ++k) rp_z[k] = a.rp_z ? a.rp_z[shot * kMaxRecPlanes + k] : -1;
+        // the maps as a buffer: the cell's byte offset in a map is co
+        const Rsrc rpm = make_rsrc(reinterpret_cast<const float*>(a.rp_map),
+                                   a.rp_map ? (long)a.nshots * kMaxRecPlanes * plane_n * 4 : 0);
 
         float q1[NF][RPT][9], q2[NF][RPT][9];
         float qn[NF][RPT];
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
 #pragma unroll
             for (int f = 0; f < NF; ++f)
 #pragma unroll
-                for (int r = 0; r < RPT; ++r) lds2[buf][f][ty + r * BY][tx] = q2[f][r][qi];
+                for (int r = 0; r < RPT; ++r) lds2[buf][f][ty + r * BY + 4][tx + 4] = q2[f][r][qi];
         };
 
         // prologue: q1 = planes z0 .. z0+8 (z0 = zb - 8; planes outside the
@@ -294,7 +299,8 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
                     for (int f = 0; f < NF; ++f) res[f] = 0.0f;
                 }
                 if (rk >= 0 && upd(r)) {  // receiver plane: inject (RTM) or record (forward)
-                    const int cidx = a.rp_map[((long)shot * kMaxRecPlanes + rk) * plane_n + (co[r] >> 2)];
+                    const unsigned moff = (unsigned)(((long)shot * kMaxRecPlanes + rk) * plane_n * 4);
+                    const int cidx = __builtin_bit_cast(int, bload<0>(rpm, co[r], moff));
                     if (cidx >= 0) {
                         if constexpr (BWD) {
                             if (a.data_row1)
@@ -312,24 +318,23 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
                     q2[f][r][8] = res[f];
                     if (pown) W1[f].template st<kNT>(wo(r), p, res[f]);
                 }
+                // one row at a time: keeps the stencil operands of only one
+                // row live
+                __builtin_amdgcn_sched_barrier(0);
             }
             // 4. step 2 at plane z
             if (main) {
 #pragma unroll
                 for (int r = 0; r < RPT; ++r) {
-                    const int lr = ty + r * BY;
-                    const int lc = tx;
-                    // lanes and rows of the margin read outside lds2; their
-                    // results are not stored, so clamp their addresses
-                    const int cr = min(max(lr, 4), TY - 5);
-                    const int cc = min(max(lc, 4), 59);
+                    const int cr = ty + r * BY + 4;
+                    const int cc = tx + 4;
                     const float vel = v2[r];
                     const float vp2 = vel * vel;
                     const float K = a.dt2 * vp2;
                     float res[NF];
 #pragma unroll
                     for (int f = 0; f < NF; ++f) {
-                        const float(*T)[64] = lds2[P][f];
+                        const float(*T)[LW] = lds2[P][f];
                         const float c = q2[f][r][4];
                         const float dxx = d2<ARITH>(c, T[cr][cc - 1], T[cr][cc + 1], T[cr][cc - 2], T[cr][cc + 2],
                                                     T[cr][cc - 3], T[cr][cc + 3], T[cr][cc - 4], T[cr][cc + 4],
@@ -361,6 +366,7 @@ __global__ void __launch_bounds__(64 * BY) step2_kernel(const Step2Args a) {
                         IL.template st<kNT>(wo(r), z, ill);
                         IM.template st<kNT>(wo(r), z, img);
                     }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);

@@ -12,7 +12,8 @@ const Entry2* find2(int mode, int arith, const Tuning& t) {
     static const Entry2 table[] = {
         FWI_E2T(FWD_RBC, ARITH_REF, 16, 2), FWI_E2T(BWD_RTM, ARITH_REF, 16, 2),
         FWI_E2T(FWD_RBC, ARITH_FP32, 16, 2), FWI_E2T(BWD_RTM, ARITH_FP32, 16, 2),
-        FWI_E2T(BWD_RTM, ARITH_REF, 12, 3), FWI_E2T(BWD_RTM, ARITH_REF, 8, 4), FWI_E2T(FWD_RBC, ARITH_REF, 12, 3),
+        FWI_E2T(FWD_RBC, ARITH_REF, 12, 3), FWI_E2T(FWD_RBC, ARITH_FP32, 12, 3), FWI_E2T(BWD_RTM, ARITH_REF, 12, 3),
+        FWI_E2T(BWD_RTM, ARITH_FP32, 12, 3),
     };
     for (const Entry2& e : table)
         if (e.mode == mode && e.arith == arith && e.by == t.by && e.rpt == t.rpt) return &e;

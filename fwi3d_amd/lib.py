@@ -39,6 +39,7 @@ EXPORTS = [
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
     "fwi_copy_bandwidth", "fwi_shot_disable", "fwi_residual_noalloc",
     "fwi_set_prefetch_depth", "fwi_get_prefetch_depth", "fwi_graph_instantiations",
+    "fwi_set_two_step", "fwi_get_two_step",
 ]
 
 
@@ -156,6 +157,8 @@ def load():
         "fwi_residual_noalloc": [P, P, ctypes.c_long, P, P],
         "fwi_set_prefetch_depth": [P, I, I],
         "fwi_get_prefetch_depth": [P, I, ctypes.POINTER(I)],
+        "fwi_set_two_step": [P, I, I, I, I, I],
+        "fwi_get_two_step": [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
         "fwi_graph_instantiations": [P, ctypes.POINTER(I)],
     }
     for name, args in sig.items():
@@ -374,6 +377,18 @@ class Context:
         v = ctypes.c_int()
         self._chk(self.L.fwi_get_prefetch_depth(self.h, kind, ctypes.byref(v)))
         return v.value
+
+    def set_two_step(self, kind, enable=True, by=16, rpt=2, zchunks=1):
+        """Two time steps per launch for kind 0 (random-boundary forward) or
+        2 (RTM backward)."""
+        self._chk(self.L.fwi_set_two_step(self.h, kind, int(enable), by, rpt, zchunks))
+
+    def two_step(self, kind):
+        """(active, by, rpt, zchunks); active 1 = used by the next pass,
+        -1 = requested but not available, 0 = off."""
+        v = [ctypes.c_int() for _ in range(4)]
+        self._chk(self.L.fwi_get_two_step(self.h, kind, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def graph_instantiations(self):
         v = ctypes.c_int()

@@ -269,6 +269,24 @@ int fwi_set_tuning(fwi_ctx* ctx, int kind, int by, int rpt, int zchunks);
 int fwi_set_prefetch_depth(fwi_ctx* ctx, int kind, int pd);
 int fwi_get_prefetch_depth(fwi_ctx* ctx, int kind, int* pd);
 
+/* Two time steps per launch (temporal blocking) for the random-boundary
+ * forward pass (kind 0) and the RTM backward pass (kind 2); replaces pairs of
+ * the per-step launches of modeling.cu:314-321 (compute_pressure) and
+ * migration.cu:72-80 (compute_pressure + cross_correlation), with the same
+ * results bit for bit.  A block of by x 64 threads (rpt rows each) computes
+ * the first step on a 64 x (by*rpt) region and the second on its inner
+ * 56 x (by*rpt - 8) cells; tiles: 16 x 2 and 12 x 3.  The forward pass
+ * records the seismogram of both steps, the RTM pass injects the data of the
+ * second step and adds both steps' image and illumination terms, all inside
+ * the launch.  Used where available: one shot per block, volumes below 2 GiB,
+ * each slot's receivers on at most four planes, and (RTM) no correction
+ * flags; elsewhere, and for the last step of an odd step range, the one-step
+ * launches run.  fwi_autotune chooses it when it is faster.
+ * fwi_get_two_step: active = 1 when the next pass of that kind uses it,
+ * -1 when requested but not available, 0 when off. */
+int fwi_set_two_step(fwi_ctx* ctx, int kind, int enable, int by, int rpt, int zchunks);
+int fwi_get_two_step(fwi_ctx* ctx, int kind, int* active, int* by, int* rpt, int* zchunks);
+
 /* Capture the per-step launches of a full pass in a hipGraph (1) or launch
  * them on the stream (0, default).  A pass's graph is instantiated once and
  * replayed by later passes of the same kind and step range, until a setter
