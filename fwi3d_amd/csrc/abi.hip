@@ -77,6 +77,10 @@ struct fwi_ctx {
     int* rp_map = nullptr;
     bool rp_ok = false;
     std::vector<long> cell_off_h;        // host copy of cell_off
+    // the 4-cell rims of the source-side / receiver-side field buffers are
+    // zero (never updated; two-step launches rely on it, since their output
+    // buffers rotate through four)
+    bool f_rims_zero = true, r_rims_zero = true;
     float* seis_tm = nullptr;            // [t][q] over all slots' receivers
     float* data_tm = nullptr;            // [t][q] backward data
     float* seis_tmp = nullptr;           // 2 * nt * nrec_total scratch for transposes
@@ -383,6 +387,7 @@ bool two_step_eligible(const fwi_ctx* c, int mode) {
     if (t.tb != 2 || !c->rp_ok || t.sb != 1 || t.big) return false;
     if (mode == fwi::BWD_RTM && c->phys != 0) return false;
     if (mode != fwi::FWD_RBC && mode != fwi::BWD_RTM) return false;
+    if (!c->f_rims_zero || (mode == fwi::BWD_RTM && !c->r_rims_zero)) return false;
     return fwi::step2_supported(mode, c->arith, two_step_tuning(t));
 }
 
@@ -1153,8 +1158,10 @@ int fwi_forward(fwi_ctx* c, int boundary) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(check_ready_forward(c, boundary));
     const long vols = c->L.vol * c->nshots;
-    HIPCHK(c, hipMemsetAsync(c->f[0], 0, vols * sizeof(float), c->st));
-    HIPCHK(c, hipMemsetAsync(c->f[1], 0, vols * sizeof(float), c->st));
+    for (float* p : c->f)
+        if (p) HIPCHK(c, hipMemsetAsync(p, 0, vols * sizeof(float), c->st));
+    if (!c->f_rims_zero) c->gen++;
+    c->f_rims_zero = true;
     if (c->nrec_total > 0)
         HIPCHK(c, hipMemsetAsync(c->seis_tm, 0, (long)c->g.nt * c->nrec_total * sizeof(float), c->st));
     c->fcur = 0;
@@ -1239,8 +1246,10 @@ static int full_backward(fwi_ctx* c, int kind) {
     CHKRC(sync_geometry(c));
     CHKRC(ensure_backward_buffers(c, kind));
     const long vols = c->L.vol * c->nshots;
-    HIPCHK(c, hipMemsetAsync(c->r[0], 0, vols * sizeof(float), c->st));
-    HIPCHK(c, hipMemsetAsync(c->r[1], 0, vols * sizeof(float), c->st));
+    for (float* p : c->r)
+        if (p) HIPCHK(c, hipMemsetAsync(p, 0, vols * sizeof(float), c->st));
+    if (!c->r_rims_zero) c->gen++;
+    c->r_rims_zero = true;
     c->rcur = 0;
     c->rold = 1;
     const int N = c->g.nt + c->g.tlag;
@@ -1280,6 +1289,19 @@ int fwi_shot_get_volume(fwi_ctx* c, int slot, int which, float* out) {
 
 int fwi_get_volume(fwi_ctx* c, int which, float* out) { return fwi_shot_get_volume(c, 0, which, out); }
 
+// Any non-zero value in the 4-cell rim of a host-layout volume (z fastest,
+// then x, then y)?
+bool rim_nonzero(const fwi_grid& g, const float* v) {
+    for (int y = 0; y < g.nyy; ++y)
+        for (int x = 0; x < g.nxx; ++x) {
+            const float* row = v + ((long)y * g.nxx + x) * g.nzz;
+            const bool edge = y < 4 || y >= g.nyy - 4 || x < 4 || x >= g.nxx - 4;
+            for (int z = 0; z < g.nzz; z = (edge || z < 3 || z >= g.nzz - 5) ? z + 1 : g.nzz - 4)
+                if (row[z] != 0.0f) return true;
+        }
+    return false;
+}
+
 int fwi_set_volume(fwi_ctx* c, int which, const float* in) {
     if (!c || !in) return c ? fail(c, FWI_ERR_ARG, "NULL input") : FWI_ERR_ARG;
     if (which == FWI_VOL_MODEL) return fwi_set_model(c, in);
@@ -1291,6 +1313,11 @@ int fwi_set_volume(fwi_ctx* c, int which, const float* in) {
     HIPCHK(c, fwi::launch_to_device(c->staging, p, c->L, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (which == FWI_VOL_FIELD || which == FWI_VOL_FIELD_OLD) c->fwd_done = 1;
+    if (rim_nonzero(c->g, in)) {
+        c->gen++;  // the next passes may run other launches: recapture pass graphs
+        if (which == FWI_VOL_FIELD || which == FWI_VOL_FIELD_OLD) c->f_rims_zero = false;
+        if (which == FWI_VOL_RECV || which == FWI_VOL_RECV_OLD) c->r_rims_zero = false;
+    }
     return FWI_OK;
 }
 

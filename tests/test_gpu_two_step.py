@@ -248,3 +248,35 @@ def test_graph_replay(fl):
             c.rtm_backward()
             assert np.array_equal(c.get_volume(fl.VOL_IMAGE), image)
         assert c.graph_instantiations() == 2
+
+
+def test_nonzero_rims_run_one_step(fl):
+    """A field state with non-zero values in its 4-cell rim (which no launch
+    updates) switches the pass to one-step launches, whose buffers keep
+    those values; a new forward pass clears the fields and the two-step
+    launches resume."""
+    case = _case()
+    with case.context(fl.Context) as c:
+        c.forward_steps(fl.BOUNDARY_RBC, 0, 20)
+        P = c.get_volume(fl.VOL_FIELD)
+        Po = c.get_volume(fl.VOL_FIELD_OLD)
+    P[0, :, :] = 1.0
+    Po[:, :, -1] = -0.5
+    outs = []
+    for two in (False, True):
+        with case.context(fl.Context) as c:
+            c.set_two_step(0, two, 16, 2, 1)
+            c.set_volume(fl.VOL_FIELD, P)
+            c.set_volume(fl.VOL_FIELD_OLD, Po)
+            c.forward_steps(fl.BOUNDARY_RBC, 20, 40)
+            if two:
+                assert c.two_step(0)[0] == -1
+            outs.append((c.get_volume(fl.VOL_FIELD), c.get_volume(fl.VOL_FIELD_OLD)))
+            if two:
+                c.forward(fl.BOUNDARY_RBC)
+                assert c.two_step(0)[0] == 1
+                _, _, seis = _oracle_forward(case)
+                assert np.array_equal(c.seismogram(), seis)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert outs[0][0][0, 5, 5] == 1.0
