@@ -78,22 +78,27 @@ def problem(v_true, nb, dt, nt, fmax, nshots_side=4, seed=0):
     return w, learned.observed(w, v_true)
 
 
-def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", n_train=None, train_offset=0):
+def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", n_train=None, train_offset=0,
+        baselines=None):
     """phase "all": train, then compare plain and learned FWI; "train": train
     only (save_net: where to store the weights, a float32 .npy; with
     load_net, training continues from those weights, so a large training set
     can be worked through in parts: n_train models starting at model
-    train_offset); "fwi": the comparison with the weights of load_net."""
+    train_offset); "fwi": the comparison with the weights of load_net.  The
+    comparison can also run in two parts: "baselines" (plain and guarded
+    plain FWI, no network) and "learned" (learned FWI with the weights of
+    load_net, against the result dict of a "baselines" run passed as
+    `baselines`)."""
     p = SCALES[scale]
     nx, ny, nz = p["grid"]
     tnx, tny, tnz = p.get("train_grid", p["grid"])
     log = log or (lambda m: None)
     out = {"scale": scale, "grid": [nx, ny, nz], "nb": p["nb"], "dt": p["dt"], "nt": p["nt"], "fmax": p["fmax"],
-           "shots": 16, "train_grid": [tnx, tny, tnz]}
+           "shots": 16, "train_grid": [tnx, tny, tnz], "seed": seed}
 
     from .unet import UNet
-    net = UNet(seed=seed + 7)
-    if load_net:
+    net = UNet(seed=seed + 7) if phase != "baselines" else None
+    if load_net and net is not None:
         net.params = np.load(load_net).astype(np.float32)
         out["network"] = "loaded"
     if phase in ("all", "train"):
@@ -129,16 +134,25 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", 
     def model_err(v):
         return float(np.linalg.norm(v - v_true) / np.linalg.norm(v_true))
 
-    v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
-    out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
-                    "initial_model_rel_l2": model_err(v0)}
-    # plain FWI with the learned loop's acceptance rule (a step is kept only
-    # when the misfit drops, else halved): the baseline without the network
-    # but with the same guard
-    t_guard = []
-    v_guard, hist_guard, sec_guard, steps_guard = fwi.guarded_fwi(prob, v0, 5, log=log, times=t_guard)
-    out["guarded_plain"] = {"misfit": hist_guard, "seconds": round(sec_guard, 3),
-                            "model_rel_l2": model_err(v_guard), "steps": steps_guard}
+    if phase == "learned":
+        # the baselines of an earlier run on the same problem
+        out["plain"], out["guarded_plain"] = baselines["plain"], baselines["guarded_plain"]
+        hist_plain, sec_plain = out["plain"]["misfit"], out["plain"]["seconds"]
+        hist_guard, sec_guard = out["guarded_plain"]["misfit"], out["guarded_plain"]["seconds"]
+    else:
+        v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
+        out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
+                        "initial_model_rel_l2": model_err(v0)}
+        # plain FWI with the learned loop's acceptance rule (a step is kept
+        # only when the misfit drops, else halved): the baseline without the
+        # network but with the same guard
+        t_guard = []
+        v_guard, hist_guard, sec_guard, steps_guard = fwi.guarded_fwi(prob, v0, 5, log=log, times=t_guard)
+        out["guarded_plain"] = {"misfit": hist_guard, "seconds": round(sec_guard, 3),
+                                "model_rel_l2": model_err(v_guard), "steps": steps_guard}
+    if phase == "baselines":
+        prob.close()
+        return out
     targets = {"plain": hist_plain[-1], "guarded_plain": hist_guard[-1]}
     t_learned = []
     v_l, hist_l, sec_l, used = learned.learned_fwi(prob, net, v0, target_misfit=min(targets.values()),
@@ -172,16 +186,26 @@ def main(argv=None):
     ap.add_argument("--scale", choices=sorted(SCALES), default="small")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
-    ap.add_argument("--phase", choices=["all", "train", "fwi"], default="all")
+    ap.add_argument("--phase", choices=["all", "train", "fwi", "baselines", "learned"], default="all")
+    ap.add_argument("--baselines", default="", help="phase learned: the JSON output of a --phase baselines run")
     ap.add_argument("--save-net", default="", help="phase train/all: store the trained weights (.npy)")
     ap.add_argument("--load-net", default="", help="weights to use (phase fwi) or to continue training from (.npy)")
     ap.add_argument("--n-train", type=int, default=None, help="training models (default: the scale's)")
     ap.add_argument("--train-offset", type=int, default=0, help="index of the first training model")
     a = ap.parse_args(argv)
-    if a.phase == "fwi" and not a.load_net:
-        ap.error("--phase fwi needs --load-net")
+    if a.phase in ("fwi", "learned") and not a.load_net:
+        ap.error("--phase %s needs --load-net" % a.phase)
+    if a.phase == "learned" and not a.baselines:
+        ap.error("--phase learned needs --baselines")
+    base = None
+    if a.baselines:
+        with open(a.baselines) as f:
+            base = json.loads(f.read())
+        if base.get("scale") != a.scale or base.get("seed", a.seed) != a.seed:
+            ap.error("--baselines is a run of another scale or seed")
     res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True), phase=a.phase,
-              save_net=a.save_net, load_net=a.load_net, n_train=a.n_train, train_offset=a.train_offset)
+              save_net=a.save_net, load_net=a.load_net, n_train=a.n_train, train_offset=a.train_offset,
+              baselines=base)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:
