@@ -28,7 +28,9 @@
 //        for the x and y derivatives of step 2.
 // The old field of step 2 is q1's plane z (the step-1 stencil field).  Eight
 // iterations before the chunk's first plane compute step 1 only (priming);
-// step-1 values outside the updated interior are zero, as the fields there.
+// step-1 values outside the updated interior are zero, as the fields there
+// (the host uses these launches only while the fields' 4-cell rims are zero:
+// the output buffers rotate through four, whose rims nothing writes).
 //
 // Between the two steps, as the one-step launches do between launches: the
 // forward pass adds the source term of the second step to the step-1 field
