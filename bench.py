@@ -373,14 +373,19 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
         kinds = {"fwd": lib.KIND_FWD_RBC, "rtm": lib.KIND_BWD_RTM}
         for item in tuning.split(";"):
             name, vals = item.split("=")
-            by, rpt, zc, sb = [int(v) for v in vals.split(",")]
-            c.set_shots_per_block(kinds[name.strip()], sb)
-            c.set_tuning(kinds[name.strip()], by, rpt, zc)
+            vals = [int(v) for v in vals.split(",")]
+            by, rpt, zc, sb = vals[:4]
+            if len(vals) > 4 and vals[4] == 2:  # shot pairs: two slots per thread
+                c.set_shots_per_thread(kinds[name.strip()], 2, by, zc)
+            else:
+                c.set_shots_per_block(kinds[name.strip()], sb)
+                c.set_tuning(kinds[name.strip()], by, rpt, zc)
     else:
         c.autotune(lib.KIND_FWD_RBC, 3)
         c.autotune(lib.KIND_BWD_RTM, 3)
-    log("batched tuning %.1f s: forward %s sb=%d, rtm %s sb=%d" % (
-        time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.tuning(2), c.shots_per_block(2)))
+    log("batched tuning %.1f s: forward %s sb=%d spt=%d, rtm %s sb=%d spt=%d" % (
+        time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.shots_per_thread(0), c.tuning(2),
+        c.shots_per_block(2), c.shots_per_thread(2)))
     if pass_steps:
         for s in range(S):
             c.set_backward_data(np.zeros((len(w16.receivers[s]), w.nt), np.float32), slot=s)
@@ -390,6 +395,8 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
         c.close()
         return {"shots": S, "profiling_pass_steps": pass_steps}
     sb_rtm = c.shots_per_block(lib.KIND_BWD_RTM)
+    shape = {"forward": list(c.tuning(0)) + [c.shots_per_block(0), c.shots_per_thread(0)],
+             "rtm": list(c.tuning(2)) + [c.shots_per_block(2), c.shots_per_thread(2)]}
     c.set_graph(graph)  # each pass's time loop as one hipGraph, instantiated once and replayed
     c.forward(lib.BOUNDARY_RBC)
     data = [c.seismogram(slot=s) for s in range(S)]
@@ -433,7 +440,17 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     rel = float(np.linalg.norm((img_b - img_s).astype(np.float64)) / np.linalg.norm(img_s.astype(np.float64)))
     log("per-shot: %.2f s; image rel-L2 batched vs per-shot %.3g" % (t_seq, rel))
     cells = (w.nxx - 8) * (w.nyy - 8) * (w.nzz - 8)
-    return {"shots": S, "graph": graph, "graph_instantiations": graphs, "sb_rtm": sb_rtm, "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
+    # the passes' own rates at the algorithmic bytes (16 B per cell-update;
+    # the backward pass's time includes the receiver injections and the
+    # pending-imaging flush, so these bound the step kernels' rates below)
+    fwd_gbs = 16.0 * cells * w.steps * S / (f_ms * 1e-3) / 1e9
+    bwd_gbs = 32.0 * cells * w.steps * S / (b_ms * 1e-3) / 1e9
+    return {"shots": S, "graph": graph, "graph_instantiations": graphs, "sb_rtm": sb_rtm,
+            "launch_shape": {"by_rpt_zchunks_sb_spt": shape},
+            "rtm_pass_us_per_shot_step": round(b_ms * 1e3 / w.steps / S, 2),
+            "rtm_pass_GBs": round(bwd_gbs, 1), "rtm_pass_frac": round(bwd_gbs / HBM_PEAK_GBS, 4),
+            "forward_pass_GBs": round(fwd_gbs, 1), "forward_pass_frac": round(fwd_gbs / HBM_PEAK_GBS, 4),
+            "shots_per_hour_batched": round(3600.0 * S / t_batch, 1),
             "shots_per_hour_per_shot": round(3600.0 * S / t_seq, 1),
             "speedup": round(t_seq / t_batch, 3),
             "gcell_upd_per_s_batched": round(cells * w.steps * 3 * S / t_batch / 1e9, 2),
@@ -470,8 +487,8 @@ def main():
                     help="also run plain FWI against learned-preconditioned FWI (time to equal misfit) at this "
                          "grid scale (fwi3d_amd.experiment)")
     ap.add_argument("--batch-tuning", default="",
-                    help="fixed batched launch shapes instead of autotuning, e.g. 'fwd=8,2,16,1;rtm=4,2,1,4' "
-                         "(BY, RPT, z-chunks, shots per block)")
+                    help="fixed batched launch shapes instead of autotuning, e.g. 'fwd=8,2,16,1;rtm=16,1,1,2,2' "
+                         "(BY, RPT, z-chunks, shots per block[, shots per thread: 2 = shot pairs])")
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="run the batched passes' time loops as hipGraphs instantiated once (1) or launch on the "
                          "stream (0)")

@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 HIP_SOURCES = ["abi.hip", "kernels_step.hip", "step_inst_fwd.hip", "step_inst_rtm.hip", "step_inst_fwi.hip",
-               "step_inst_batch.hip", "step_inst_prefetch.hip", "step_inst_big.hip", "step_inst_two.hip",
+               "step_inst_batch.hip", "step_inst_prefetch.hip", "step_inst_big.hip", "step_inst_two.hip", "step_inst_pair.hip",
                "kernels_aux.hip", "unet_kernels.hip", "unet_net.hip", "noalloc.hip"]
 
 CXX = shutil.which("g++") or "g++"

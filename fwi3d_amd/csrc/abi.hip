@@ -357,6 +357,9 @@ StepArgs base_args(fwi_ctx* c) {
     a.sstride = c->L.vol;
     a.nshots = c->nshots;
     a.vp = c->vp;
+    // every slot's model is slot 0's (fwi_set_model / _interior replicate
+    // it) except within nb of a face, where the random boundaries write
+    a.share_nb = c->g.nb;
     a.istride = c->L.vol;
     a.idh2 = c->idh2;
     a.dt2 = c->dt2;
@@ -492,11 +495,28 @@ int run_forward(fwi_ctx* c, int boundary, int n0, int n1, bool record = true) {
     return FWI_OK;
 }
 
+// Shot pairs address both shots' volumes through one buffer resource with
+// 32-bit offsets: two volumes must stay below 2 GiB.
+bool pairs_fit(const fwi_ctx* c) { return 2 * c->L.vol * (long)sizeof(float) < (1L << 31); }
+
+// The tuning of a backward pass: the two-shots-per-thread instances carry no
+// corrections, so with corrections a pass of shot pairs runs the
+// two-groups-per-block instances (the same summation order) on 64 x 4 tiles.
+Tuning backward_tuning(const fwi_ctx* c, int mode) {
+    Tuning t = c->tune[mode];
+    if (t.spt == 2 && c->phys != 0) {
+        t.spt = 1;
+        t.by = 4;
+        t.rpt = 1;
+    }
+    return t;
+}
+
 // Backward steps [b0, b1) of every slot from the current state (receiver
 // data injected before each step b < nt, in receiver order per cell).
 int run_backward(fwi_ctx* c, int kind, int b0, int b1) {
     const int mode = kind == 0 ? fwi::BWD_RTM : fwi::BWD_FWI;
-    const Tuning& t = c->tune[mode];
+    const Tuning t = backward_tuning(c, mode);
     const int nt = c->g.nt;
     StepArgs a = base_args(c);
     a.image = kind == 0 ? c->image : c->grad;
@@ -964,6 +984,7 @@ int fwi_set_arith(fwi_ctx* c, int arith) {
             c->tune[k].by = 4;
             c->tune[k].rpt = 2;
             c->tune[k].sb = 1;
+            c->tune[k].spt = 1;
             c->tune[k].pd = 1;
         }
     c->arith = arith;
@@ -1483,6 +1504,7 @@ int fwi_set_shots_per_block(fwi_ctx* c, int kind, int sb) {
     if (kind < 0 || kind > 3) return fail(c, FWI_ERR_ARG, "kind");
     Tuning t = c->tune[kind];
     t.sb = sb;
+    t.spt = 1;  // one shot per thread: sb groups of waves
     if (sb != 1) t.pd = 1;  // two-plane prefetch: one shot per block only
     if (sb < 1 || c->nshots % sb != 0) return fail(c, FWI_ERR_ARG, "unsupported shots-per-block");
     // keep the current tile if it exists for this sb, else take the first one that does
@@ -1503,6 +1525,33 @@ int fwi_get_shots_per_block(fwi_ctx* c, int kind, int* sb) {
     return FWI_OK;
 }
 
+int fwi_set_shots_per_thread(fwi_ctx* c, int kind, int spt, int by, int zchunks) {
+    if (!c) return FWI_ERR_ARG;
+    c->gen++;  // launch arguments may change: recapture pass graphs
+    if (kind < 0 || kind > 3 || (spt != 1 && spt != 2)) return fail(c, FWI_ERR_ARG, "kind / shots per thread");
+    if (zchunks < 1 || zchunks > c->g.nzz - 8) return fail(c, FWI_ERR_ARG, "z chunks");
+    Tuning t = c->tune[kind];
+    t.spt = spt;
+    t.by = by;
+    t.zchunks = zchunks;
+    t.pd = 1;
+    if (spt == 2) {
+        if (c->nshots % 2 != 0) return fail(c, FWI_ERR_ARG, "shot pairs need an even number of slots");
+        if (!pairs_fit(c)) return fail(c, FWI_ERR_RANGE, "shot pairs need two volumes below 2 GiB");
+        t.sb = 2;
+        t.rpt = 1;
+    }
+    if (!fwi::step_config_supported(kind, c->arith, t)) return fail(c, FWI_ERR_ARG, "unsupported launch shape");
+    c->tune[kind] = t;
+    return FWI_OK;
+}
+
+int fwi_get_shots_per_thread(fwi_ctx* c, int kind, int* spt) {
+    if (!c || kind < 0 || kind > 3 || !spt) return FWI_ERR_ARG;
+    *spt = c->tune[kind].spt;
+    return FWI_OK;
+}
+
 int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     if (!c) return FWI_ERR_ARG;
     c->gen++;  // launch arguments may change: recapture pass graphs
@@ -1511,7 +1560,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     HIPCHK(c, hipSetDevice(c->device));
     CHKRC(sync_geometry(c));
     char key[200];
-    std::snprintf(key, sizeof key, "step-tb kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith,
+    std::snprintf(key, sizeof key, "step-spt kind=%d arith=%d nxx=%d nyy=%d nzz=%d nshots=%d", kind, c->arith,
                   c->g.nxx, c->g.nyy, c->g.nzz, c->nshots);
     if (cache_path) {
         std::ifstream in(cache_path);
@@ -1522,7 +1571,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
             std::istringstream ss(line.substr(bar + 1));
             Tuning t;
             t.big = c->tune[kind].big;
-            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd >> t.tb >> t.tby >> t.trpt >> t.tzc &&
+            if (ss >> t.by >> t.rpt >> t.zchunks >> t.sb >> t.pd >> t.tb >> t.tby >> t.trpt >> t.tzc >> t.spt &&
                 c->nshots % t.sb == 0 && fwi::step_config_supported(kind, c->arith, t) &&
                 (t.tb == 1 || fwi::step2_supported(kind, c->arith, two_step_tuning(t)))) {
                 c->tune[kind] = t;
@@ -1543,12 +1592,31 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     const bool saved_graph = c->use_graph, saved_kt = c->ktime;
     c->use_graph = false;
     c->ktime = false;
-    double best = 1e300;
+    double best = 1e300;  // ms per step
     Tuning best_t = saved;
     const int zc_choices[] = {1, 2, 4, 8, 16};
-    const int sb_choices[] = {1, 2, 4};
+    // shots per block: 1, 2, 4 groups of waves; 2 as one shot pair per thread (sb 3 below)
+    const int sb_choices[] = {1, 2, 4, 3};
     int rc = FWI_OK;
-    for (int sb : sb_choices) {
+    // trial steps past nt: no injection, no recording, no data reads
+    auto body = [&](int nsteps) {
+        if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
+        return run_forward(c, kind == 1 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC, c->g.nt, c->g.nt + nsteps, false);
+    };
+    // a candidate's time per step over n steps (after one untimed step)
+    auto trial = [&](const Tuning& t, int n, double* ms) {
+        c->tune[kind] = t;
+        int r = timed_pass(c, 1, [&] { return body(1); });
+        if (!r) r = timed_pass(c, n, [&] { return body(n); });
+        *ms = c->last_ms / n;
+        return r;
+    };
+    // screen every candidate with a short trial, then time the four best
+    // again with four times the steps, twice, and keep the fastest: the
+    // short trials alone mis-rank shapes within a few per cent of each other
+    std::vector<std::pair<double, Tuning>> screened;
+    for (int sbc : sb_choices) {
+        const int sb = sbc == 3 ? 2 : sbc;
         if (c->nshots % sb != 0) continue;
         for (auto& tc : kTileChoices) {
             for (int pd = 1; pd <= 2; ++pd)
@@ -1558,27 +1626,34 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
                 t.rpt = tc[1];
                 t.zchunks = zc;
                 t.sb = sb;
+                t.spt = sbc == 3 ? 2 : 1;
                 t.pd = pd;
                 t.big = saved.big;
                 t.tb = 1;
                 if (!fwi::step_config_supported(kind, c->arith, t) || zc > c->g.nzz - 8) continue;
-                c->tune[kind] = t;
-                // trial steps past nt: no injection, no recording, no data reads
-                auto body = [&](int nsteps) {
-                    if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
-                    return run_forward(c, kind == 1 ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC, c->g.nt,
-                                       c->g.nt + nsteps, false);
-                };
-                if ((rc = timed_pass(c, 1, [&] { return body(1); }))) break;
-                if ((rc = timed_pass(c, steps, [&] { return body(steps); }))) break;
-                if (c->last_ms < best) {
-                    best = c->last_ms;
-                    best_t = t;
-                }
+                if (t.spt == 2 && !pairs_fit(c)) continue;
+                double ms = 0.0;
+                if ((rc = trial(t, steps, &ms))) break;
+                screened.push_back({ms, t});
             }
             if (rc) break;
         }
         if (rc) break;
+    }
+    if (!rc) {
+        std::stable_sort(screened.begin(), screened.end(),
+                         [](const std::pair<double, Tuning>& x, const std::pair<double, Tuning>& y) {
+                             return x.first < y.first;
+                         });
+        for (size_t i = 0; i < screened.size() && i < 4 && !rc; ++i)
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                double ms = 0.0;
+                if ((rc = trial(screened[i].second, 4 * steps, &ms))) break;
+                if (ms < best) {
+                    best = ms;
+                    best_t = screened[i].second;
+                }
+            }
     }
     // two-step launches (random-boundary forward, RTM backward): tiles and z
     // chunks, kept when faster than the best one-step configuration; the
@@ -1602,15 +1677,10 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
                 t.trpt = tc[1];
                 t.tzc = zc;
                 if (!fwi::step2_supported(kind, c->arith, two_step_tuning(t)) || zc > c->g.nzz - 8) continue;
-                c->tune[kind] = t;
-                auto body = [&](int nsteps) {
-                    if (bwd) return run_backward(c, kind - 2, c->g.nt, c->g.nt + nsteps);
-                    return run_forward(c, FWI_BOUNDARY_RBC, c->g.nt, c->g.nt + nsteps, false);
-                };
-                if ((rc = timed_pass(c, 1, [&] { return body(2); }))) break;
-                if ((rc = timed_pass(c, steps, [&] { return body(steps); }))) break;
-                if (c->last_ms < best) {
-                    best = c->last_ms;
+                double ms = 0.0;
+                if ((rc = trial(t, 4 * steps, &ms))) break;
+                if (ms < best) {
+                    best = ms;
                     best_t = t;
                 }
             }
@@ -1631,7 +1701,7 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
         std::ofstream out(cache_path, std::ios::app);
         out << key << "|" << best_t.by << " " << best_t.rpt << " " << best_t.zchunks << " " << best_t.sb << " "
             << best_t.pd << " " << best_t.tb << " " << best_t.tby << " " << best_t.trpt << " " << best_t.tzc << " "
-            << best << "\n";
+            << best_t.spt << " " << best << "\n";
     }
     return FWI_OK;
 }

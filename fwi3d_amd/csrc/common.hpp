@@ -66,6 +66,7 @@ struct Tuning {
     int pd = 1;       // prefetch depth: planes of loads in flight (1 or 2)
     int big = 0;      // per-plane addressing: set for volumes of 2 GiB and more
     int tb = 1;       // time steps per launch (2: two-step launches where available)
+    int spt = 1;      // shots per thread: 2 = one thread advances a cell of two shots (sb = 2, rpt = 1)
     // tile and z chunks of the two-step launches (tb = 2)
     int tby = 16, trpt = 2, tzc = 1;
 };
@@ -78,6 +79,10 @@ struct StepArgs {
     int ntx, nty, zlen, nzc;
     int nitems;            // ntx * nty * nzc * shot groups
     const float* vp;       // per shot
+    // cells x, y, z in [share_nb, n - share_nb) hold the same velocity in
+    // every shot of the launch (the random boundaries write only outside);
+    // -1: no such guarantee
+    int share_nb;
     const float* cur;      // field read by the stencil: A (forward) / So (backward)
     float* old;            // in place: previous -> next
     const float* rcur;     // backward: receiver field R

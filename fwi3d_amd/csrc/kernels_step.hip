@@ -11,19 +11,21 @@ const Entry* step_table_fwi(int* count);
 const Entry* step_table_batch(int* count);
 const Entry* step_table_prefetch(int* count);
 const Entry* step_table_big(int* count);
+const Entry* step_table_pair(int* count);
 
 namespace {
 
 const Entry* find(int mode, int arith, const Tuning& t) {
-    int counts[6];
-    const Entry* tables[] = {step_table_fwd(&counts[0]), step_table_rtm(&counts[1]), step_table_fwi(&counts[2]),
-                             step_table_batch(&counts[3]), step_table_prefetch(&counts[4]),
-                             step_table_big(&counts[5])};
-    for (int k = 0; k < 6; ++k)
+    int counts[7];
+    const Entry* tables[] = {step_table_fwd(&counts[0]),      step_table_rtm(&counts[1]),
+                             step_table_fwi(&counts[2]),      step_table_batch(&counts[3]),
+                             step_table_prefetch(&counts[4]), step_table_big(&counts[5]),
+                             step_table_pair(&counts[6])};
+    for (int k = 0; k < 7; ++k)
         for (int i = 0; i < counts[k]; ++i) {
             const Entry& e = tables[k][i];
             if (e.mode == mode && e.arith == arith && e.by == t.by && e.rpt == t.rpt && e.sb == t.sb &&
-                e.pd == t.pd && e.big == t.big)
+                e.pd == t.pd && e.big == t.big && e.spt == t.spt)
                 return &e;
         }
     return nullptr;
@@ -61,7 +63,7 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     a.nitems = a.ntx * a.nty * a.nzc * (nshots / sb);
     const int blocks = (a.nitems + 7) / 8 * 8;
     grid = dim3(blocks);
-    block = dim3(64, t.by * sb);
+    block = dim3(64, t.by * sb / t.spt);
 }
 
 bool step_defer_supported(int mode, int arith, const Tuning& t, bool corrections) {

@@ -849,6 +849,7 @@ struct Entry {
     // 4: illumination, 5: both; 4 also for FWI)
     LaunchFn fn_img3, fn_img4, fn_img5;
     LaunchFn fn_corr_img3, fn_corr_img4, fn_corr_img5;  // with the corrections (not aligned imaging)
+    int spt = 1;       // shots per thread (2: pair_kernel, sb = 2 shots per block)
 };
 
 constexpr bool is_bwd(int m) { return m == BWD_RTM || m == BWD_FWI; }

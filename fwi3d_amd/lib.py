@@ -39,7 +39,7 @@ EXPORTS = [
     "fwi_shot_residual", "fwi_shot_get_volume", "fwi_set_shots_per_block", "fwi_get_shots_per_block", "fwi_shot_get_backward_data",
     "fwi_copy_bandwidth", "fwi_shot_disable", "fwi_residual_noalloc",
     "fwi_set_prefetch_depth", "fwi_get_prefetch_depth", "fwi_graph_instantiations",
-    "fwi_set_two_step", "fwi_get_two_step",
+    "fwi_set_two_step", "fwi_get_two_step", "fwi_set_shots_per_thread", "fwi_get_shots_per_thread",
 ]
 
 
@@ -160,6 +160,8 @@ def load():
         "fwi_set_two_step": [P, I, I, I, I, I],
         "fwi_get_two_step": [P, I, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)],
         "fwi_graph_instantiations": [P, ctypes.POINTER(I)],
+        "fwi_set_shots_per_thread": [P, I, I, I, I],
+        "fwi_get_shots_per_thread": [P, I, ctypes.POINTER(I)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -290,6 +292,15 @@ class Context:
         sb = ctypes.c_int()
         self._chk(self.L.fwi_get_shots_per_block(self.h, kind, ctypes.byref(sb)))
         return sb.value
+
+    def set_shots_per_thread(self, kind, spt, by=16, zchunks=1):
+        """spt = 2: shot pairs (two consecutive slots per thread, 64 x by tiles)."""
+        self._chk(self.L.fwi_set_shots_per_thread(self.h, kind, spt, by, zchunks))
+
+    def shots_per_thread(self, kind):
+        spt = ctypes.c_int()
+        self._chk(self.L.fwi_get_shots_per_thread(self.h, kind, ctypes.byref(spt)))
+        return spt.value
 
     # -- passes -------------------------------------------------------------
     def forward(self, boundary=BOUNDARY_RBC):

@@ -248,6 +248,18 @@ int fwi_shot_get_volume(fwi_ctx* ctx, int slot, int which, float* out_padded);
 int fwi_set_shots_per_block(fwi_ctx* ctx, int kind, int sb);
 int fwi_get_shots_per_block(fwi_ctx* ctx, int kind, int* sb);
 
+/* Shots advanced by one thread: 2 = each thread advances one cell of two
+ * consecutive slots (a shot pair; nshots even), on 64 x by tiles (by 4, 8 or
+ * 16) and `zchunks` blocks along z.  The pair shares one image /
+ * illumination read-modify-write (summed in the order of two shots per
+ * block) and, in the interior, the velocity read.  1 = one shot per thread
+ * (the shots-per-block setting applies).  Reference arithmetic only; a
+ * backward pass with corrections runs pairs as two wave groups per block.
+ * Batched counterpart of the per-shot launches of modeling.cu:314-321 and
+ * migration.cu:72-80, which the reference runs one shot at a time. */
+int fwi_set_shots_per_thread(fwi_ctx* ctx, int kind, int spt, int by, int zchunks);
+int fwi_get_shots_per_thread(fwi_ctx* ctx, int kind, int* spt);
+
 /* Autotune tile shape, z-chunking and shots per block for a pass kind
  * (0 forward RBC, 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing
  * `steps` steps of each candidate (backward kinds: rounded up to a multiple

@@ -130,10 +130,26 @@ def _interleaved_oracle_rtm(case, shots, vps, datas, sb, fwi=False, phys=0):
     return fimg, fill
 
 
-@pytest.mark.parametrize("sb", [1, 2, 4])
+def _set_batch_shape(c, kinds, sb):
+    """sb shots per block as wave groups, or "pair<BY>": shot pairs (two
+    slots per thread) on 64 x BY tiles, two z-chunks."""
+    for kind in kinds:
+        if isinstance(sb, str):
+            c.set_shots_per_thread(kind, 2, int(sb[4:]), 2)
+        else:
+            c.set_shots_per_block(kind, sb)
+
+
+def _sb(sb):
+    return 2 if isinstance(sb, str) else sb
+
+
+@pytest.mark.parametrize("sb", [1, 2, 4, "pair4", "pair8", "pair16"])
 def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
-    """With sb shots per block the image and illumination equal, bit for
-    bit, the oracle run in the batched summation order."""
+    """With sb shots per block (or shot pairs per thread) the image and
+    illumination equal, bit for bit, the oracle run in the batched summation
+    order.  The slots' random boundaries differ, so a pair's shared interior
+    velocity and its own boundary velocities are both exercised."""
     case = Case(nx=40, ny=36, nz=28, nt=50, amp=0.0)
     S = 4
     shots = _shots(case, S)
@@ -146,28 +162,31 @@ def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
         c.set_source(*src, slot=s)
         c.set_receivers(rec, slot=s)
     c.set_wavelet(case.wavelet)
-    for kind in (0, 2):
-        c.set_shots_per_block(kind, sb)
-    c.set_graph(sb == 2)  # the hipGraph-captured time loop gives the same bits
+    _set_batch_shape(c, (0, 2), sb)
+    c.set_graph(sb in (2, "pair8"))  # the hipGraph-captured time loop gives the same bits
     c.forward(fl.BOUNDARY_RBC)
-    rng = np.random.default_rng(sb)
+    rng = np.random.default_rng(_sb(sb))
     datas = []
     for s in range(S):
-        d = (c.seismogram(slot=s) + rng.normal(0, 1e-3, (len(shots[s][1]), case.nt))).astype(np.float32)
+        seis = c.seismogram(slot=s)
+        _, _, want = O.forward(vps[s], case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, shots[s][0],
+                               shots[s][1])
+        assert np.array_equal(seis, want), s
+        d = (seis + rng.normal(0, 1e-3, (len(shots[s][1]), case.nt))).astype(np.float32)
         datas.append(d)
         c.set_backward_data(d, slot=s)
     c.rtm_backward()
     img_b = c.get_volume(fl.VOL_IMAGE)
     ill_b = c.get_volume(fl.VOL_ILLUM)
     c.close()
-    want_img, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, sb)
+    want_img, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, _sb(sb))
     assert np.abs(want_img).max() > 0
     assert np.array_equal(ill_b, want_ill)
     assert np.array_equal(img_b, want_img)
 
 
 @pytest.mark.parametrize("fwi", [False, True])
-@pytest.mark.parametrize("sb", [2, 4])
+@pytest.mark.parametrize("sb", [2, 4, "pair16"])
 def test_batched_deferred_imaging_identical(fl, sb, fwi):
     """Batched RTM with the image updated every other launch and the
     illumination every third (the default; the other groups hand group 0
@@ -193,12 +212,12 @@ def test_batched_deferred_imaging_identical(fl, sb, fwi):
             c.set_receivers(rec, slot=s)
         c.set_wavelet(case.wavelet)
         for kind in (0, 3 if fwi else 2):
-            c.set_shots_per_block(kind, sb)
+            _set_batch_shape(c, (kind,), sb)
             by, rpt, _ = c.tuning(kind)
             c.set_tuning(kind, by, rpt, 2)
         c.set_deferred_imaging(defer)
         c.forward(fl.BOUNDARY_RBC)
-        rng = np.random.default_rng(sb)
+        rng = np.random.default_rng(_sb(sb))
         for s in range(S):
             d = (c.seismogram(slot=s) + rng.normal(0, 1e-3, (len(shots[s][1]), case.nt))).astype(np.float32)
             c.set_backward_data(d, slot=s)
@@ -213,6 +232,65 @@ def test_batched_deferred_imaging_identical(fl, sb, fwi):
     for k in (1, 3):
         assert np.array_equal(outs[k][0], outs[k - 1][0]), k
         assert np.array_equal(outs[k][1], outs[k - 1][1]), k
+
+
+@pytest.mark.parametrize("phys", [0, 7])
+def test_shot_pair_fwi_gradient_vs_interleaved_oracle(fl, phys):
+    """FWI gradient with shot pairs: the reference physics on the pair
+    instances (gradient every step, illumination every third launch), and
+    with all corrections, which run the pairs as two wave groups per block;
+    bit for bit against the oracle in the batched summation order.  The
+    absorbing-boundary forward of the same pairs (the FWI calculated data)
+    equals the oracle's seismograms."""
+    case = Case(nx=40, ny=36, nz=28, nt=50, amp=0.0, abc_factor=0.05)
+    S = 4
+    shots = _shots(case, S)
+    padded = O.expand_boundary(case.interior, case.nb)
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    c.set_wavelet(case.wavelet)
+    c.set_dampers(case.abc[0])
+    for s, (src, rec) in enumerate(shots):
+        c.set_source(*src, slot=s)
+        c.set_receivers(rec, slot=s)
+    _set_batch_shape(c, (1,), "pair16")
+    c.forward(fl.BOUNDARY_ABC)
+    for s, (src, rec) in enumerate(shots):
+        _, _, want = O.forward(padded, case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, src, rec,
+                               abc_tables=case.abc)
+        assert np.array_equal(c.seismogram(slot=s), want), s
+    vps = [O.random_boundary(padded, case.nb, 99, s, 0.6, case.vclamp) for s in range(S)]
+    for s in range(S):
+        c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+    _set_batch_shape(c, (0, 3), "pair8")
+    c.set_physics(phys)
+    c.forward(fl.BOUNDARY_RBC)
+    rng = np.random.default_rng(11 + phys)
+    datas = []
+    for s in range(S):
+        d = rng.normal(0, 1e-2, (len(shots[s][1]), case.nt)).astype(np.float32)
+        datas.append(d)
+        c.set_backward_data(d, slot=s)
+    c.zero_volume(fl.VOL_GRAD)
+    c.grad_backward()
+    g_b = c.get_volume(fl.VOL_GRAD)
+    ill_b = c.get_volume(fl.VOL_ILLUM)
+    c.close()
+    want_g, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, 2, fwi=True, phys=phys)
+    assert np.abs(want_g).max() > 0
+    assert np.array_equal(ill_b, want_ill)
+    assert np.array_equal(g_b, want_g)
+
+
+def test_shot_pairs_need_an_even_batch(fl):
+    case = Case(nx=24, ny=24, nz=20, nt=20, amp=0.0)
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=3)
+    with pytest.raises(fl.FwiError):
+        c.set_shots_per_thread(2, 2, 16, 1)
+    with pytest.raises(fl.FwiError):
+        c.set_shots_per_thread(2, 3, 16, 1)
+    assert c.shots_per_thread(2) == 1
+    c.close()
 
 
 def test_batched_fwi_gradient_with_a_receiverless_shot(fl):
