@@ -415,8 +415,24 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     t_batch = time.perf_counter() - t0
     img_b = c.get_volume(lib.VOL_IMAGE)
     graphs = c.graph_instantiations()
+    # the batch's step kernels alone: HIP events around every launch of a
+    # 24-step segment past the end of the data (no injection or recording;
+    # a multiple of six launches, so the backward segment has the whole
+    # pass's mix of imaging cadences), after the timed passes
+    c.set_graph(False)
+    c.set_kernel_timing(True)
+    kernels = {}
+    for name, run in (("forward", lambda: c.forward_steps(lib.BOUNDARY_RBC, w.nt, w.nt + 24)),
+                      ("rtm", lambda: c.backward_steps(0, w.nt, w.nt + 24))):
+        best = None
+        for _ in range(2):
+            run()
+            ms, n = c.last_kernel_timing()
+            best = ms / n if best is None else min(best, ms / n)
+        kernels[name] = best
     c.close()
-    log("batched: %.2f s (forward %.0f ms, backward %.0f ms)" % (t_batch, f_ms, b_ms))
+    log("batched: %.2f s (forward %.0f ms, backward %.0f ms); kernels per launch: forward %.0f us, rtm %.0f us"
+        % (t_batch, f_ms, b_ms, kernels["forward"] * 1e3, kernels["rtm"] * 1e3))
 
     # per-shot path on the single-shot context
     single.zero_volume(lib.VOL_IMAGE)
@@ -445,8 +461,19 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     # pending-imaging flush, so these bound the step kernels' rates below)
     fwd_gbs = 16.0 * cells * w.steps * S / (f_ms * 1e-3) / 1e9
     bwd_gbs = 32.0 * cells * w.steps * S / (b_ms * 1e-3) / 1e9
+    rtm_k = 32.0 * cells * S / (kernels["rtm"] * 1e-3) / 1e9
+    fwd_k = 16.0 * cells * S / (kernels["forward"] * 1e-3) / 1e9
     return {"shots": S, "graph": graph, "graph_instantiations": graphs, "sb_rtm": sb_rtm,
             "launch_shape": {"by_rpt_zchunks_sb_spt": shape},
+            "roofline": {"bound": "hbm", "kernel": "batched RTM backward step (%d shots per launch)" % S,
+                         "achieved": round(rtm_k, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(rtm_k / HBM_PEAK_GBS, 4),
+                         "avg_launch_us": round(kernels["rtm"] * 1e3, 1),
+                         "us_per_shot_step": round(kernels["rtm"] * 1e3 / S, 2),
+                         "algorithmic_bytes_per_launch": 32 * cells * S},
+            "forward_kernel": {"avg_launch_us": round(kernels["forward"] * 1e3, 1),
+                               "us_per_shot_step": round(kernels["forward"] * 1e3 / S, 2),
+                               "achieved_GBs": round(fwd_k, 1), "frac": round(fwd_k / HBM_PEAK_GBS, 4)},
             "rtm_pass_us_per_shot_step": round(b_ms * 1e3 / w.steps / S, 2),
             "rtm_pass_GBs": round(bwd_gbs, 1), "rtm_pass_frac": round(bwd_gbs / HBM_PEAK_GBS, 4),
             "forward_pass_GBs": round(fwd_gbs, 1), "forward_pass_frac": round(fwd_gbs / HBM_PEAK_GBS, 4),

@@ -60,8 +60,10 @@ void step_grid(const Layout& L, const Tuning& t, int nshots, int sb, StepArgs& a
     const int M = L.nz - 8;
     a.zlen = (M + t.zchunks - 1) / t.zchunks;
     a.nzc = (M + a.zlen - 1) / a.zlen;
-    a.nitems = a.ntx * a.nty * a.nzc * (nshots / sb);
-    const int blocks = (a.nitems + 7) / 8 * 8;
+    const int per_grp = a.ntx * a.nty * a.nzc;
+    a.nitems = per_grp * (nshots / sb);
+    // the kernels' item order: 8 blocks per tile range of an XCD, per group
+    const int blocks = (per_grp + 7) / 8 * 8 * (nshots / sb);
     grid = dim3(blocks);
     block = dim3(64, t.by * sb / t.spt);
 }

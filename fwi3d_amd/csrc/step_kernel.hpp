@@ -295,17 +295,23 @@ __global__ void __launch_bounds__(64 * BY * SB) step_kernel(const StepArgs a) {
     const long vol_b = L.vol * 4;                  // bytes of one volume (< 2 GiB)
     const unsigned plane_b = (unsigned)(L.pp * 4);  // bytes of one z-plane
 
-    // XCD-aware item order: workgroups are dispatched round-robin over the
-    // 8 XCDs, so blocks b, b+8, b+16, ... share an L2.  Give each XCD a
-    // contiguous range of items (adjacent tiles share halo rows).
-    const int nblk = gridDim.x;
+    // Item order (step_grid launches 8 * range blocks per shot group):
+    // workgroups are dispatched round-robin over the 8 XCDs, so blocks b,
+    // b+8, b+16, ... share an L2, and in block order.  The shot groups (the
+    // slowest index of an item) are taken one after another by the whole
+    // GPU, and within a group each XCD takes a contiguous range of tiles, so
+    // the blocks running at any time hold neighbouring tiles of one group,
+    // whose halo rows meet in L2.
+    const int per_grp = a.ntx * a.nty * a.nzc;
+    const int range = (per_grp + 7) >> 3;
     const int xcd = blockIdx.x & 7;
-    const int slot = blockIdx.x >> 3;
-    const int per_xcd_blocks = nblk >> 3;
-    const int range = (a.nitems + 7) >> 3;
-    const int item_end = min((xcd + 1) * range, a.nitems);
+    const int nslots = range * (a.nitems / per_grp);
 
-    for (int item = xcd * range + slot; item < item_end; item += per_xcd_blocks) {
+    for (int slot = blockIdx.x >> 3; slot < nslots; slot += gridDim.x >> 3) {
+        const int gq = slot / range;
+        const int t = xcd * range + (slot - gq * range);
+        if (t >= per_grp) continue;  // uniform per block
+        const int item = t + gq * per_grp;
         const int tile_x = item % a.ntx;
         const int r1 = item / a.ntx;
         const int tile_y = r1 % a.nty;
