@@ -1582,6 +1582,15 @@ int fwi_autotune(fwi_ctx* c, int kind, int steps, const char* cache_path) {
     }
     const bool bwd = kind >= 2;
     if (bwd) CHKRC(ensure_backward_buffers(c, kind - 2));
+    // the trials run on noise, not on zero fields: a wavefield's changing
+    // bits draw more power and lower the clock, and the shapes rank
+    // differently on zeros (the buffers are cleared after the trials)
+    {
+        const long vols = c->L.vol * c->nshots;
+        uint64_t k = 1;
+        for (float* p : {c->f[0], c->f[1], c->f[2], c->f[3], c->r[0], c->r[1], c->r[2], c->r[3]})
+            if (p) HIPCHK(c, fwi::launch_fill_noise(p, vols, 0x5EED0000ull + k++, c->st));
+    }
     // backward trials time a multiple of six steps: with deferred imaging a
     // pass repeats the image update every other launch and the illumination
     // update every second or third, and another count would add updates at

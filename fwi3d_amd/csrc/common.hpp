@@ -177,6 +177,7 @@ hipError_t launch_blob_boundary(float* dev, const Layout& L, int nb, float dh, u
 hipError_t launch_add_points(float* dev, const long* off, int n, long sstride, float value, hipStream_t st);
 hipError_t launch_fold_groups(float* v, long vol, int groups, hipStream_t st);
 hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st);
+hipError_t launch_fill_noise(float* v, long n, uint64_t seed, hipStream_t st);
 hipError_t launch_flush_imaging(const float* cur, const float* old, const float* rold, float* image, float* illum,
                                 const Layout& L, int nshots, int sb, long istride, int ill_terms, int img_term,
                                 hipStream_t st);

@@ -567,6 +567,21 @@ __global__ void __launch_bounds__(256) copy_kernel(const copy_vec* __restrict__ 
         if (base + u * 256 < n) __builtin_nontemporal_store(v[u], dst + base + u * 256);
 }
 
+// Trial data of the autotuner: hashed values uniform in [-1, 1), so that
+// the trials run on fields whose bits change the way a wavefield's do (the
+// step kernels draw less power, and run faster, on zero fields).
+__global__ void __launch_bounds__(256) fill_noise_kernel(float* __restrict__ v, long n, uint64_t seed) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const uint64_t h = splitmix64(seed ^ ((uint64_t)i * 0x9E3779B97F4A7C15ull));
+        v[i] = (float)(int)(h >> 40) * (1.0f / 8388608.0f) - 1.0f;
+    }
+}
+
+hipError_t launch_fill_noise(float* v, long n, uint64_t seed, hipStream_t st) {
+    fill_noise_kernel<<<4096, 256, 0, st>>>(v, n, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const float* src, float* dst, long nfloats, hipStream_t st) {
     const long n = nfloats / 4;
     copy_kernel<<<(unsigned)((n + 1023) / 1024), 256, 0, st>>>((const copy_vec*)src, (copy_vec*)dst, n);

@@ -56,7 +56,12 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
     constexpr int LW = 64 + 8;
     constexpr int LH = BY + 8;
     constexpr int HY = (8 + BY - 1) / BY;  // y-halo rows loaded per wave
-    __shared__ float lds[2][2 * NF][LH][LW];
+    // centre planes of both shots, double buffered: [buffer][shot][row][col],
+    // each entry a cell's NF fields side by side, so that one 8-byte LDS
+    // access moves both fields and every neighbour of a thread is an
+    // immediate offset from one address
+    using CellT = typename std::conditional<NF == 2, float2, float>::type;
+    __shared__ CellT lds[2][2][LH][LW];
 
     const Layout& L = a.L;
     const int tx = threadIdx.x;
@@ -209,19 +214,32 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
             if constexpr (RMW_IMG) sim[S] = bload<kNT>(ir, co, soff(zz, 0));
             if constexpr (RMW_ILL) sil[S] = bload<kNT>(lr_, co, soff(zz, 0));
         };
+        auto cell_of = [&](const float(&v)[NF]) {
+            if constexpr (NF == 2)
+                return make_float2(v[0], v[1]);
+            else
+                return v[0];
+        };
         auto stage_plane = [&](int buf, int qi) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s) {
+                CellT(*T)[LW] = lds[buf][s];
+                float cv[NF];
 #pragma unroll
-                for (int f = 0; f < NF; ++f) {
-                    float(*T)[LW] = lds[buf][s * NF + f];
-                    const float cv = q[s][f][qi];
-                    T[ty + 4][tx + 4] = cv;
-                    T[ty + 4][hx_lane ? hxc : tx + 4] = hx_lane ? hx[s][f] : cv;
+                for (int f = 0; f < NF; ++f) cv[f] = q[s][f][qi];
+                // x halos without a per-lane branch: a lane with no halo
+                // cell rewrites its own centre value
+                T[ty + 4][tx + 4] = cell_of(cv);
+                T[ty + 4][hx_lane ? hxc : tx + 4] = hx_lane ? cell_of(hx[s]) : cell_of(cv);
 #pragma unroll
-                    for (int m = 0; m < HY; ++m)
-                        if (ty + m * BY < 8) T[hy_row[m]][tx + 4] = hy[s][f][m];
-                }
+                for (int m = 0; m < HY; ++m)
+                    if (ty + m * BY < 8) {
+                        float hv[NF];
+#pragma unroll
+                        for (int f = 0; f < NF; ++f) hv[f] = hy[s][f][m];
+                        T[hy_row[m]][tx + 4] = cell_of(hv);
+                    }
+            }
         };
 
         // prologue: planes zb-4 .. zb+4 into the queues, plane zb into LDS
@@ -264,12 +282,25 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const int lr = ty + 4, lc = tx + 4;
-                const float(*Pt)[LW] = lds[P][s * NF];
+                const CellT(*T)[LW] = lds[P][s];
+                // the 16 x and y neighbours, both fields each
+                const CellT xm1 = T[lr][lc - 1], xp1 = T[lr][lc + 1], xm2 = T[lr][lc - 2], xp2 = T[lr][lc + 2];
+                const CellT xm3 = T[lr][lc - 3], xp3 = T[lr][lc + 3], xm4 = T[lr][lc - 4], xp4 = T[lr][lc + 4];
+                const CellT ym1 = T[lr - 1][lc], yp1 = T[lr + 1][lc], ym2 = T[lr - 2][lc], yp2 = T[lr + 2][lc];
+                const CellT ym3 = T[lr - 3][lc], yp3 = T[lr + 3][lc], ym4 = T[lr - 4][lc], yp4 = T[lr + 4][lc];
+                auto fx = [](const CellT& v, auto fc) {
+                    if constexpr (NF == 2)
+                        return decltype(fc)::value == 0 ? v.x : v.y;
+                    else
+                        return v;
+                };
                 const float c = q[s][0][4];
-                const float dxx = d2<ARITH>(c, Pt[lr][lc - 1], Pt[lr][lc + 1], Pt[lr][lc - 2], Pt[lr][lc + 2],
-                                            Pt[lr][lc - 3], Pt[lr][lc + 3], Pt[lr][lc - 4], Pt[lr][lc + 4], a.idh2);
-                const float dyy = d2<ARITH>(c, Pt[lr - 1][lc], Pt[lr + 1][lc], Pt[lr - 2][lc], Pt[lr + 2][lc],
-                                            Pt[lr - 3][lc], Pt[lr + 3][lc], Pt[lr - 4][lc], Pt[lr + 4][lc], a.idh2);
+                const float dxx = d2<ARITH>(c, fx(xm1, Parity<0>{}), fx(xp1, Parity<0>{}), fx(xm2, Parity<0>{}),
+                                            fx(xp2, Parity<0>{}), fx(xm3, Parity<0>{}), fx(xp3, Parity<0>{}),
+                                            fx(xm4, Parity<0>{}), fx(xp4, Parity<0>{}), a.idh2);
+                const float dyy = d2<ARITH>(c, fx(ym1, Parity<0>{}), fx(yp1, Parity<0>{}), fx(ym2, Parity<0>{}),
+                                            fx(yp2, Parity<0>{}), fx(ym3, Parity<0>{}), fx(yp3, Parity<0>{}),
+                                            fx(ym4, Parity<0>{}), fx(yp4, Parity<0>{}), a.idh2);
                 const float* qz = q[s][0];
                 const float dzz = d2<ARITH>(c, qz[3], qz[5], qz[2], qz[6], qz[1], qz[7], qz[0], qz[8], a.idh2);
                 const float vel = s == 0 ? sv0[SC] : v_pair1;
@@ -293,12 +324,13 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
                     if (z == src_z[s] && cell == src_c[s]) nv = nv + a.inj;
                     res[s][0] = nv;
                 } else {
-                    const float(*R)[LW] = lds[P][s * NF + 1];
                     const float cr = q[s][1][4];
-                    const float rxx = d2<ARITH>(cr, R[lr][lc - 1], R[lr][lc + 1], R[lr][lc - 2], R[lr][lc + 2],
-                                                R[lr][lc - 3], R[lr][lc + 3], R[lr][lc - 4], R[lr][lc + 4], a.idh2);
-                    const float ryy = d2<ARITH>(cr, R[lr - 1][lc], R[lr + 1][lc], R[lr - 2][lc], R[lr + 2][lc],
-                                                R[lr - 3][lc], R[lr + 3][lc], R[lr - 4][lc], R[lr + 4][lc], a.idh2);
+                    const float rxx = d2<ARITH>(cr, fx(xm1, Parity<1>{}), fx(xp1, Parity<1>{}), fx(xm2, Parity<1>{}),
+                                                fx(xp2, Parity<1>{}), fx(xm3, Parity<1>{}), fx(xp3, Parity<1>{}),
+                                                fx(xm4, Parity<1>{}), fx(xp4, Parity<1>{}), a.idh2);
+                    const float ryy = d2<ARITH>(cr, fx(ym1, Parity<1>{}), fx(yp1, Parity<1>{}), fx(ym2, Parity<1>{}),
+                                                fx(yp2, Parity<1>{}), fx(ym3, Parity<1>{}), fx(yp3, Parity<1>{}),
+                                                fx(ym4, Parity<1>{}), fx(yp4, Parity<1>{}), a.idh2);
                     const float* rz = q[s][1];
                     const float rzz = d2<ARITH>(cr, rz[3], rz[5], rz[2], rz[6], rz[1], rz[7], rz[0], rz[8], a.idh2);
                     const float o = so[SC][s];
@@ -326,6 +358,9 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
                     p1[s] = c * c;
                     pimg[s] = c * sro[SC][s];
                 }
+                // one shot at a time: its 16 neighbour reads are live only
+                // during its own derivatives
+                __builtin_amdgcn_sched_barrier(0);
             }
             // image and illumination: each step's terms in shot order, the
             // steps in order

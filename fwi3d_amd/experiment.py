@@ -79,7 +79,7 @@ def problem(v_true, nb, dt, nt, fmax, nshots_side=4, seed=0):
 
 
 def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", n_train=None, train_offset=0,
-        baselines=None):
+        baselines=None, which=("plain", "guarded_plain"), tune=True):
     """phase "all": train, then compare plain and learned FWI; "train": train
     only (save_net: where to store the weights, a float32 .npy; with
     load_net, training continues from those weights, so a large training set
@@ -88,13 +88,14 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", 
     comparison can also run in two parts: "baselines" (plain and guarded
     plain FWI, no network) and "learned" (learned FWI with the weights of
     load_net, against the result dict of a "baselines" run passed as
-    `baselines`)."""
+    `baselines`); `which` names the baselines a "baselines" run computes
+    (both by default; one each keeps a large run within one GPU session)."""
     p = SCALES[scale]
     nx, ny, nz = p["grid"]
     tnx, tny, tnz = p.get("train_grid", p["grid"])
     log = log or (lambda m: None)
     out = {"scale": scale, "grid": [nx, ny, nz], "nb": p["nb"], "dt": p["dt"], "nt": p["nt"], "fmax": p["fmax"],
-           "shots": 16, "train_grid": [tnx, tny, tnz], "seed": seed}
+           "shots": 16, "train_grid": [tnx, tny, tnz], "seed": seed, "tuned": tune}
 
     from .unet import UNet
     net = UNet(seed=seed + 7) if phase != "baselines" else None
@@ -128,7 +129,7 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", 
     v_true = true_model(nx, ny, nz, seed=seed + 1)
     v0 = learned.smooth(v_true, p["sigma"])
     w, obs = problem(v_true, p["nb"], p["dt"], p["nt"], p["fmax"], seed=seed)
-    prob = fwi.FwiProblem(w, obs, zmask=20.0)
+    prob = fwi.FwiProblem(w, obs, zmask=20.0, tune=tune)
     prob.misfit(v0)  # warm-up: first launches, autotune caches
 
     def model_err(v):
@@ -140,16 +141,18 @@ def run(scale="small", seed=0, log=None, phase="all", save_net="", load_net="", 
         hist_plain, sec_plain = out["plain"]["misfit"], out["plain"]["seconds"]
         hist_guard, sec_guard = out["guarded_plain"]["misfit"], out["guarded_plain"]["seconds"]
     else:
-        v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
-        out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3), "model_rel_l2": model_err(v_plain),
-                        "initial_model_rel_l2": model_err(v0)}
-        # plain FWI with the learned loop's acceptance rule (a step is kept
-        # only when the misfit drops, else halved): the baseline without the
-        # network but with the same guard
-        t_guard = []
-        v_guard, hist_guard, sec_guard, steps_guard = fwi.guarded_fwi(prob, v0, 5, log=log, times=t_guard)
-        out["guarded_plain"] = {"misfit": hist_guard, "seconds": round(sec_guard, 3),
-                                "model_rel_l2": model_err(v_guard), "steps": steps_guard}
+        if "plain" in which or phase != "baselines":
+            v_plain, hist_plain, sec_plain = fwi.plain_fwi(prob, v0, 5, log=log)
+            out["plain"] = {"misfit": hist_plain, "seconds": round(sec_plain, 3),
+                            "model_rel_l2": model_err(v_plain), "initial_model_rel_l2": model_err(v0)}
+        if "guarded_plain" in which or phase != "baselines":
+            # plain FWI with the learned loop's acceptance rule (a step is
+            # kept only when the misfit drops, else halved): the baseline
+            # without the network but with the same guard
+            t_guard = []
+            v_guard, hist_guard, sec_guard, steps_guard = fwi.guarded_fwi(prob, v0, 5, log=log, times=t_guard)
+            out["guarded_plain"] = {"misfit": hist_guard, "seconds": round(sec_guard, 3),
+                                    "model_rel_l2": model_err(v_guard), "steps": steps_guard}
     if phase == "baselines":
         prob.close()
         return out
@@ -187,7 +190,14 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default="")
     ap.add_argument("--phase", choices=["all", "train", "fwi", "baselines", "learned"], default="all")
-    ap.add_argument("--baselines", default="", help="phase learned: the JSON output of a --phase baselines run")
+    ap.add_argument("--baselines", default="",
+                    help="phase learned: the JSON output of a --phase baselines run (or of several, comma-separated, "
+                         "whose baselines are merged)")
+    ap.add_argument("--untuned", action="store_true",
+                    help="run the FWI passes with the default launch shape instead of autotuned forward shapes "
+                         "and the fixed 64 x 16 backward tiles")
+    ap.add_argument("--which", default="plain,guarded_plain",
+                    help="phase baselines: the baselines to run (plain, guarded_plain)")
     ap.add_argument("--save-net", default="", help="phase train/all: store the trained weights (.npy)")
     ap.add_argument("--load-net", default="", help="weights to use (phase fwi) or to continue training from (.npy)")
     ap.add_argument("--n-train", type=int, default=None, help="training models (default: the scale's)")
@@ -199,13 +209,18 @@ def main(argv=None):
         ap.error("--phase learned needs --baselines")
     base = None
     if a.baselines:
-        with open(a.baselines) as f:
-            base = json.loads(f.read())
-        if base.get("scale") != a.scale or base.get("seed", a.seed) != a.seed:
-            ap.error("--baselines is a run of another scale or seed")
+        for path in a.baselines.split(","):
+            with open(path) as f:
+                part = json.loads(f.read())
+            if part.get("scale") != a.scale or part.get("seed", a.seed) != a.seed:
+                ap.error("--baselines is a run of another scale or seed")
+            base = part if base is None else dict(base, **{k: v for k, v in part.items()
+                                                            if k in ("plain", "guarded_plain")})
+        if "plain" not in base or "guarded_plain" not in base:
+            ap.error("--baselines must hold both the plain and the guarded plain run")
     res = run(a.scale, a.seed, log=lambda m: print(m, file=sys.stderr, flush=True), phase=a.phase,
               save_net=a.save_net, load_net=a.load_net, n_train=a.n_train, train_offset=a.train_offset,
-              baselines=base)
+              baselines=base, which=tuple(a.which.split(",")), tune=not a.untuned)
     s = json.dumps(res)
     print(s, flush=True)
     if a.out:

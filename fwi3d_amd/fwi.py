@@ -9,6 +9,8 @@ m = 1/v^2 update.  All shots of the problem run as one batch.  This module
 drives the configuration-3/4 experiments and generates the learned
 preconditioner's training triples.
 """
+import os
+import tempfile
 import time
 
 import numpy as np
@@ -28,9 +30,21 @@ def crop(vol, nb):
 
 
 class FwiProblem:
-    """Observed data, geometry and the two batch contexts of one inversion."""
+    """Observed data, geometry and the two batch contexts of one inversion.
 
-    def __init__(self, w, obs, nb_abc=None, nb_rbc=None, abc_factor=0.004, zmask=40.0, seed=None, device=0):
+    tune: autotune the launch shapes of the forward passes (absorbing and
+    random boundary) when the contexts are built; the choices are cached in
+    `tune_cache` (default: one file per process in the temporary
+    directory), keyed by grid and shot count, so that problems of the same
+    size reuse them (forward results do not depend on the launch shape).
+    The backward pass sums the shots' gradient and illumination terms in an
+    order that depends on how shots share a block, so its shape is fixed
+    rather than timed: one shot per thread and block (each shot its own
+    partial sums, folded after the pass, as with the default shape) on
+    64 x 16 tiles, so that a seed fixes every result bit for bit."""
+
+    def __init__(self, w, obs, nb_abc=None, nb_rbc=None, abc_factor=0.004, zmask=40.0, seed=None, device=0,
+                 tune=True, tune_cache=None):
         self.w = w
         self.obs = [np.ascontiguousarray(o, np.float32) for o in obs]
         self.S = len(w.shots)
@@ -41,6 +55,11 @@ class FwiProblem:
         self.abc = self._context(self.nb_abc, device)
         self.abc.set_dampers(b1d_table(self.nb_abc, abc_factor))
         self.rbc = self._context(self.nb_rbc, device)
+        if tune:
+            cache = tune_cache or os.path.join(tempfile.gettempdir(), "fwi3d_autotune_%d.txt" % os.getpid())
+            self.abc.autotune(lib.KIND_FWD_ABC, 3, cache)
+            self.rbc.autotune(lib.KIND_FWD_RBC, 3, cache)
+            self.rbc.set_tuning(lib.KIND_BWD_FWI, 16, 1, 1)
         self.forward_passes = 0
         self.seconds = {"misfit": 0.0, "gradient": 0.0}
 
