@@ -113,6 +113,7 @@ void Inversion::compute_gradient() {
         stage_info = "Computing synthetic data";
         show_inv_info();
         place_shots(ctx, abc_nb, srcId, count);
+        tune(ctx, 0, 1);
         check(fwi_forward(ctx, FWI_BOUNDARY_ABC), "fwi_forward");  // get_cal_data, inversion.cu:178-188
         for (int slot = 0; slot < count; slot++) {                  // set_adjoint_source, :190-201
             set_obs_data(srcId + slot);
@@ -129,6 +130,8 @@ void Inversion::compute_gradient() {
             check(fwi_shot_get_backward_data(ctx, slot, residual.data()), "fwi_shot_get_backward_data");
             check(fwi_shot_set_backward_data(rbc, slot, residual.data()), "fwi_shot_set_backward_data");
         }
+        tune(rbc, 1, 0);
+        tune(rbc, 1, 3);
         check(fwi_forward(rbc, FWI_BOUNDARY_RBC), "fwi_forward");  // :218-233
         check(fwi_grad_backward(rbc), "fwi_grad_backward");        // :235-253
     }
@@ -160,6 +163,7 @@ double Inversion::model_misfit(const std::vector<float>& velocity) {
         const int count = std::min(batch, geometry->nsrc - srcId);
         show_inv_info();
         place_shots(ctx, abc_nb, srcId, count);
+        tune(ctx, 0, 1);
         check(fwi_forward(ctx, FWI_BOUNDARY_ABC), "fwi_forward");
         for (int slot = 0; slot < count; slot++) {
             set_obs_data(srcId + slot);

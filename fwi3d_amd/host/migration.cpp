@@ -40,6 +40,9 @@ void Migration::forward_propagation() {
     show_mig_info();
     initialization();
     for (int slot = 0; slot < shot_count; slot++) random_boundary(ctx, slot, first_shot + slot, rbc_ratio, rbc_varVp);
+    // the backward shape is chosen before the first forward pass: the
+    // autotuner's trials overwrite the fields that pass leaves
+    tune(ctx, 0, 2);
     forward_solver();
 }
 

@@ -62,6 +62,12 @@ void Modeling::set_build_parameters() {
     if (a == "reference") arith = FWI_ARITH_REFERENCE;
     else if (a == "fp32") arith = FWI_ARITH_FP32;
     else throw std::invalid_argument("arithmetic must be 'reference' or 'fp32'");
+    const std::string at = catch_parameter_or("autotune", parameters, "off");
+    if (at == "off") autotune = 0;
+    else if (at == "on") autotune = 1;
+    else if (at == "all") autotune = 2;
+    else throw std::invalid_argument("autotune must be 'off', 'on' or 'all', not '" + at + "'");
+    autotune_cache = catch_parameter_or("autotune_cache", parameters, "");
     physics = 0;
     std::string corr = catch_parameter_or("backward_corrections", parameters, "none");
     size_t p0 = 0;
@@ -150,6 +156,21 @@ void Modeling::create_context(int boundary_cells, fwi_ctx** out) {
 
 float Modeling::cfl_velocity() const { return 0.45f * dh / dt; }
 
+// The autotune key (not a reference parameter): launch shapes timed by
+// fwi_autotune before a context's first pass of a kind.  Forward results do
+// not depend on the shape; backward image / gradient sums do, through the
+// grouping of shots, so "on" fixes the backward shape instead.
+void Modeling::tune(fwi_ctx* c, int context, int kind) {
+    const unsigned bit = 1u << (kind + 4 * context);
+    if (autotune == 0 || (tuned_kinds & bit)) return;
+    tuned_kinds |= bit;
+    if (autotune == 1 && kind >= 2) {
+        check(fwi_set_tuning(c, kind, 16, 1, 1), "fwi_set_tuning");
+        return;
+    }
+    check(fwi_autotune(c, kind, 3, autotune_cache.empty() ? nullptr : autotune_cache.c_str()), "fwi_autotune");
+}
+
 // modeling.cu:558-592 (blobs) or the ramp boundary, seeded by (rbc_seed, shot)
 void Modeling::random_boundary(fwi_ctx* c, int slot, int shot, float ratio, float varVp) {
     if (rbc_blobs)
@@ -189,6 +210,7 @@ void Modeling::initialization() {
 
 // modeling.cu:309-322
 void Modeling::forward_solver() {
+    tune(ctx, 0, ABC ? 1 : 0);
     check(fwi_forward(ctx, ABC ? FWI_BOUNDARY_ABC : FWI_BOUNDARY_RBC), "fwi_forward");
 }
 

@@ -63,6 +63,13 @@ protected:
     bool rbc_blobs = true;  // rbc_style
     int arith = FWI_ARITH_REFERENCE;
     int physics = 0;  // FWI_PHYS_* bits for the backward passes
+    // autotune key: 0 off (default launch shapes), 1 forward passes timed,
+    // backward passes on fixed one-shot 64 x 16 tiles (the summation order
+    // of the default shape), 2 every pass kind timed, shots per block or
+    // thread included
+    int autotune = 0;
+    std::string autotune_cache;     // optional cache file of the choices
+    unsigned tuned_kinds = 0;       // bit (kind + 4 * context): done
 
     void set_wavelet();
     void set_geometry();
@@ -75,6 +82,8 @@ protected:
 
     void create_context(int boundary_cells, fwi_ctx** out);
     void check(int rc, const std::string& what) const;
+    // once per context and pass kind, before its first pass (autotune key)
+    void tune(fwi_ctx* c, int context, int kind);
     void grid_index(float x, float y, float z, int* ix, int* iy, int* iz) const;
     float cfl_velocity() const;
     // the random boundary of rbc_style on slot `slot` of c for shot `shot`

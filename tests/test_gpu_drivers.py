@@ -232,3 +232,40 @@ def test_full_workflow_in_cpp(tiny, tmp_path):
     out = _run("inversion_main", params)
     conv = np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_2_iterations.txt"))
     assert conv.shape == (3,) and conv[-1] < conv[0], out[-1500:]
+
+
+def test_drivers_autotune_key(tiny, tmp_path):
+    """The drivers' optional `autotune` key.  "on": the forward passes take
+    the autotuner's launch shapes and the backward passes fixed one-shot
+    tiles, so the seismograms, the RTM image and the FWI convergence curve
+    equal the default run's bit for bit; "all": every pass kind is timed
+    (shots per block or per thread included), the image sums the shots in
+    the chosen grouping and stays within rounding of the default run."""
+    from scipy.ndimage import gaussian_filter
+    w = tiny
+    out = {}
+    for mode in ("off", "on", "all"):
+        folder = str(tmp_path / mode)
+        params = w.write(folder, batch_shots=2)
+        with open(params, "a") as f:
+            f.write("autotune = %s\nautotune_cache = %s\n" % (mode, os.path.join(folder, "tune.txt")))
+        _run("modeling_main", params)
+        _run("migration_main", params)
+        img = np.fromfile(os.path.join(folder, "outputs/seismic", "RTM_section_45Hz_%dx%dx%d_10m.bin"
+                                       % (w.nz, w.nx, w.ny)), np.float32)
+        seis = [_seis(folder, w, s) for s in range(len(w.shots))]
+        init = (1.0 / gaussian_filter(1.0 / w.interior.astype(np.float64), 2.0)).astype(np.float32)
+        fparams = w.write(folder, model=init, kind="init", max_iteration=1, batch_shots=2, depth_mask=20.0)
+        with open(fparams, "a") as f:
+            f.write("autotune = %s\n" % mode)
+        _run("inversion_main", fparams)
+        conv = np.loadtxt(os.path.join(folder, "outputs/residuo", "convergence_1_iterations.txt"))
+        out[mode] = (seis, img, conv)
+        if mode != "off":
+            assert os.path.getsize(os.path.join(folder, "tune.txt")) > 0
+    for s in range(len(w.shots)):
+        assert np.array_equal(out["on"][0][s], out["off"][0][s]), s
+        assert np.array_equal(out["all"][0][s], out["off"][0][s]), s
+    assert np.array_equal(out["on"][1], out["off"][1])
+    assert np.array_equal(out["on"][2], out["off"][2])
+    assert rel_l2(out["all"][1], out["off"][1]) <= 1e-4
