@@ -111,11 +111,14 @@ class FwiProblem:
 
 
 def parabola_step(a0, a1, a2, f0, f1, f2):
-    """Vertex of the parabola through three points, float32 as inversion.cu:304-312."""
+    """Vertex of the parabola through three points, float32 as inversion.cu:304-312.
+    Collinear points (den = 0) give a non-finite step, as in the reference;
+    the callers take the better trial step then."""
     a0, a1, a2, f0, f1, f2 = (np.float32(v) for v in (a0, a1, a2, f0, f1, f2))
     num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2
     den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2
-    return float(np.float32(0.5) * (num / den))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return float(np.float32(0.5) * (num / den))
 
 
 def velocity_bounds(w, vlo=500.0):
