@@ -25,10 +25,10 @@
 //    folded image are the same bits.
 //
 // HBM bytes per shot, cell and step (compulsory): forward 16 B with the
-// velocity read for both shots, 14.3 B at the 401 x 401 x 201 (+50) grid
+// velocity read for both shots, 15.1 B at the 401 x 401 x 201 (+50) grid
 // where 43 % of the cells are interior; RTM backward 28 B of fields and
-// velocity (26.3 B with the shared velocity) plus the image/illumination
-// read-modify-writes, 6.7 B per step on average, halved by the pair: 29.6 B
+// velocity (27.1 B with the shared velocity) plus the image/illumination
+// read-modify-writes, 6.7 B per step on average, halved by the pair: 30.5 B
 // against 34.7 B for one shot per thread.
 //
 // Instances: reference arithmetic, no backward corrections (a batched pass
