@@ -260,12 +260,18 @@ int fwi_get_shots_per_block(fwi_ctx* ctx, int kind, int* sb);
 int fwi_set_shots_per_thread(fwi_ctx* ctx, int kind, int spt, int by, int zchunks);
 int fwi_get_shots_per_thread(fwi_ctx* ctx, int kind, int* spt);
 
-/* Autotune tile shape, z-chunking and shots per block for a pass kind
- * (0 forward RBC, 1 forward ABC, 2 RTM backward, 3 FWI backward) by timing
- * `steps` steps of each candidate (backward kinds: rounded up to a multiple
- * of six, the deferred-imaging pattern of whole passes); cache_path (may be
- * NULL) persists the choice.  The trial steps run on the context's buffers, which are zeroed
- * afterwards: autotune before setting up a pass. */
+/* Autotune tile shape, z-chunking, prefetch depth and shots per block or
+ * per thread for a pass kind (0 forward RBC, 1 forward ABC, 2 RTM backward,
+ * 3 FWI backward) by timing `steps` steps of each candidate (backward
+ * kinds: rounded up to a multiple of six, the deferred-imaging pattern of
+ * whole passes), then the four fastest again with four times the steps;
+ * cache_path (may be NULL) persists the choice.  The trial steps run on the
+ * context's field buffers filled with noise (the kernels draw more power,
+ * and rank differently, on non-zero fields); every field, image,
+ * illumination and gradient buffer is zeroed afterwards: autotune before
+ * setting up a pass.  Choosing several shots per block or per thread for a
+ * backward kind changes the order in which the shots' image terms are
+ * summed (equal within rounding). */
 int fwi_autotune(fwi_ctx* ctx, int kind, int steps, const char* cache_path);
 
 /* Current launch configuration: tile rows per block, rows per thread,
