@@ -90,8 +90,8 @@ __global__ void __launch_bounds__(64 * BY, 4) pair_kernel(const StepArgs a) {
     // b+8, b+16, ... share an L2, and in block order.  The shot groups (the
     // slowest index of an item) are taken one after another by the whole
     // GPU, and within a group each XCD takes a contiguous range of tiles, so
-    // the blocks running at any time hold neighbouring tiles of one group,
-    // whose halo rows meet in L2.
+    // the blocks running at any time hold neighbouring tiles of one group
+    // (whose halo rows they share) rather than tiles of several groups.
     const int per_grp = a.ntx * a.nty * a.nzc;
     const int range = (per_grp + 7) >> 3;
     const int xcd = blockIdx.x & 7;
