@@ -144,21 +144,47 @@ def _sb(sb):
     return 2 if isinstance(sb, str) else sb
 
 
-@pytest.mark.parametrize("sb", [1, 2, 4, "pair4", "pair8", "pair16"])
-def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb):
+@pytest.mark.parametrize(
+    "sb,boundary",
+    [(1, "ramp"), (2, "ramp"), (4, "ramp"), ("pair4", "ramp"), ("pair8", "ramp"), ("pair16", "ramp"),
+     (2, "blobs"), ("pair8", "blobs")],
+)
+def test_batched_rtm_bit_exact_vs_interleaved_oracle(fl, sb, boundary):
     """With sb shots per block (or shot pairs per thread) the image and
     illumination equal, bit for bit, the oracle run in the batched summation
     order.  The slots' random boundaries differ, so a pair's shared interior
-    velocity and its own boundary velocities are both exercised."""
+    velocity and its own boundary velocities are both exercised: seeded
+    ramps, or the reference-style velocity blobs of the migration and
+    inversion drivers (modeling.cu:558-615)."""
     case = Case(nx=40, ny=36, nz=28, nt=50, amp=0.0)
     S = 4
     shots = _shots(case, S)
     padded = O.expand_boundary(case.interior, case.nb)
-    vps = [O.random_boundary(padded, case.nb, 99, s, 0.6, case.vclamp) for s in range(S)]
+    ratio, var_vp = 40.0, 800.0
+    if boundary == "ramp":
+        vps = [O.random_boundary(padded, case.nb, 99, s, 0.6, case.vclamp) for s in range(S)]
+    else:
+        vlo = np.float32(case.interior.min()) - np.float32(var_vp)
+        vhi = np.float32(case.interior.max()) + np.float32(var_vp)
+        vps = []
+        for s in range(S):
+            blobs = fl.rbc_blobs(7, s, case.nxx, case.nyy, case.nzz, case.nb, case.dh, ratio, var_vp)
+            vps.append(O.random_boundary_blobs(padded, case.nb, case.dh, 7, s, var_vp, blobs, vlo, vhi,
+                                               case.vclamp))
+    # what a pair's shared interior velocity rests on: the boundaries differ,
+    # the interiors do not
+    nb = case.nb
+    for s in range(1, S):
+        assert not np.array_equal(vps[s], vps[0])
+        assert np.array_equal(vps[s][nb:-nb, nb:-nb, nb:-nb], vps[0][nb:-nb, nb:-nb, nb:-nb])
     c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
     c.set_model_interior(case.interior)
     for s, (src, rec) in enumerate(shots):
-        c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+        if boundary == "ramp":
+            c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+        else:
+            c.random_boundary_blobs(7, s, ratio, var_vp, float(case.vclamp), slot=s)
+        assert np.array_equal(c.get_volume(fl.VOL_MODEL, slot=s), vps[s]), s
         c.set_source(*src, slot=s)
         c.set_receivers(rec, slot=s)
     c.set_wavelet(case.wavelet)
