@@ -106,11 +106,13 @@ def test_config1_forward_and_rtm_segments_bit_exact(fl):
         assert np.array_equal(a, b), name
 
 
-def test_config2_survey_batch_bit_exact_vs_oracle(fl):
+@pytest.mark.parametrize("shape", ["autotuned", "pairs"])
+def test_config2_survey_batch_bit_exact_vs_oracle(fl, shape):
     """The 16-shot 4 x 4 survey (receiver patches clipped at the edges: 1024
     to 1936 receivers per shot) on a 101 x 101 x 51 gradient model with a
-    20-cell boundary, autotuned tile and shots per block, graph-captured
-    passes: seismograms bit for bit against per-shot oracle runs, image and
+    20-cell boundary, autotuned tile and shots per block (or shot pairs on
+    64 x 8 tiles, two z-chunks, for both passes), graph-captured passes:
+    seismograms bit for bit against per-shot oracle runs, image and
     illumination bit for bit against the oracle in the batch's order."""
     from fwi3d_amd import configs
     w = configs.gradient_survey()
@@ -123,9 +125,14 @@ def test_config2_survey_batch_bit_exact_vs_oracle(fl):
         c.shot_random_boundary(s, w.rbc_seed, s, w.rbc_amp, w.vclamp)
         c.set_source(*w.src_index(s), slot=s)
         c.set_receivers(w.rec_index(s), slot=s)
-    c.autotune(fl.KIND_FWD_RBC, 3)
-    c.autotune(fl.KIND_BWD_RTM, 3)
+    if shape == "pairs":
+        c.set_shots_per_thread(fl.KIND_FWD_RBC, 2, 8, 2)
+        c.set_shots_per_thread(fl.KIND_BWD_RTM, 2, 8, 2)
+    else:
+        c.autotune(fl.KIND_FWD_RBC, 3)
+        c.autotune(fl.KIND_BWD_RTM, 3)
     sb = c.shots_per_block(fl.KIND_BWD_RTM)
+    assert shape != "pairs" or (sb == 2 and c.shots_per_thread(fl.KIND_BWD_RTM) == 2)
     print("config 2 tuning: forward %s sb=%d, rtm %s sb=%d"
           % (c.tuning(0), c.shots_per_block(0), c.tuning(2), sb))
     c.set_graph(True)
