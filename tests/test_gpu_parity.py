@@ -89,6 +89,52 @@ def test_backward_bit_exact(fl, fwi):
     assert np.array_equal(g_Ro, Ro)
 
 
+@pytest.mark.parametrize("fwi,two_step", [(False, False), (True, False), (False, True)])
+def test_colocated_receivers_bit_exact(fl, fwi, two_step):
+    """Receivers sharing a cell (the reference injects them with atomics,
+    migration.cu:119-132, inversion.cu:385-398): each records the cell, and
+    the backward pass adds their data in receiver order, as the oracle does
+    (also inside the two-step launches, which record and inject through a
+    map of the receiver planes).  The duplicates sit far apart in the list,
+    one cell holds three receivers, and one receiver sits on the source
+    cell."""
+    case = Case(nx=84, ny=60, nz=31, nt=61) if two_step else Case()
+    rec = case.rec
+    extra = np.concatenate([rec[::7], rec[len(rec) // 2:len(rec) // 2 + 1], [case.src]]).astype(np.int32)
+    rec = np.concatenate([rec[len(rec) // 3:], extra, rec[:len(rec) // 3], rec[len(rec) // 2:len(rec) // 2 + 1]])
+    cells = [tuple(r) for r in rec]
+    assert max(cells.count(c) for c in cells) == 3
+    case.rec = np.ascontiguousarray(rec, np.int32)
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(5)
+    data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+    S, So, R, Ro, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold, fwi=fwi,
+                                            tlag=case.tlag)
+    with case.context(fl.Context) as c:
+        if two_step:
+            c.set_two_step(fl.KIND_FWD_RBC, True, 16, 2, 1)
+            c.set_two_step(fl.KIND_BWD_RTM, True, 16, 2, 1)
+        c.forward(fl.BOUNDARY_RBC)
+        g_seis = c.seismogram()
+        c.set_backward_data(data)
+        (c.grad_backward if fwi else c.rtm_backward)()
+        if two_step:
+            assert c.two_step(fl.KIND_FWD_RBC)[0] == 1 and c.two_step(fl.KIND_BWD_RTM)[0] == 1
+        g_img = c.get_volume(fl.VOL_GRAD if fwi else fl.VOL_IMAGE)
+        g_ill = c.get_volume(fl.VOL_ILLUM)
+        g_R = c.get_volume(fl.VOL_RECV)
+        g_Ro = c.get_volume(fl.VOL_RECV_OLD)
+    assert np.array_equal(g_seis, seis)
+    for i, ci in enumerate(cells):  # co-located receivers record the same trace
+        j = cells.index(ci)
+        assert np.array_equal(g_seis[i], g_seis[j])
+    assert np.abs(image).max() > 0
+    assert np.array_equal(g_R, R)
+    assert np.array_equal(g_Ro, Ro)
+    assert np.array_equal(g_ill, illum)
+    assert np.array_equal(g_img, image)
+
+
 @pytest.mark.parametrize("phys", [0, 3, 7])
 @pytest.mark.parametrize("arith", ["reference", "fp32"])
 @pytest.mark.parametrize("fwi", [False, True])
