@@ -308,6 +308,50 @@ def test_shot_pair_fwi_gradient_vs_interleaved_oracle(fl, phys):
     assert np.array_equal(g_b, want_g)
 
 
+@pytest.mark.parametrize("sb", [2, "pair8"])
+def test_batched_sources_and_receivers_on_the_updated_edge(fl, sb):
+    """A batch whose shots sit on opposite corners of the updated region
+    ((4, 4, 4) and its far corner) with receivers on the corners and faces
+    next to the rims: seismograms against per-shot oracle runs, image and
+    illumination against the oracle in the batch's order, bit for bit."""
+    case = Case(nt=60, amp=0.0)
+    hi = (case.nxx - 5, case.nyy - 5, case.nzz - 5)
+    rec = np.array([(4, 4, 4), hi, (4, hi[1], 4), (hi[0], 4, hi[2]), (4, 4, hi[2]), (hi[0], hi[1], 4),
+                    (case.nxx // 2, 4, case.nzz // 2), (4, case.nyy // 2, case.nzz // 2),
+                    (case.nxx // 2, case.nyy // 2, hi[2]), (5, 4, 4), (hi[0] - 1, hi[1], hi[2])], np.int32)
+    shots = [((4, 4, 4), rec), (hi, rec[::-1].copy())]
+    S = len(shots)
+    padded = O.expand_boundary(case.interior, case.nb)
+    vps = [O.random_boundary(padded, case.nb, 99, s, 0.6, case.vclamp) for s in range(S)]
+    c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=S)
+    c.set_model_interior(case.interior)
+    for s, (src, r) in enumerate(shots):
+        c.shot_random_boundary(s, 99, s, 0.6, float(case.vclamp))
+        c.set_source(*src, slot=s)
+        c.set_receivers(r, slot=s)
+    c.set_wavelet(case.wavelet)
+    _set_batch_shape(c, (0, 2), sb)
+    c.forward(fl.BOUNDARY_RBC)
+    rng = np.random.default_rng(13)
+    datas = []
+    for s in range(S):
+        seis = c.seismogram(slot=s)
+        _, _, want = O.forward(vps[s], case.nb, case.dh, case.dt, case.nt, case.tlag, case.wavelet, shots[s][0],
+                               shots[s][1])
+        assert np.array_equal(seis, want), s
+        d = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+        datas.append(d)
+        c.set_backward_data(d, slot=s)
+    c.rtm_backward()
+    img_b = c.get_volume(fl.VOL_IMAGE)
+    ill_b = c.get_volume(fl.VOL_ILLUM)
+    c.close()
+    want_img, want_ill = _interleaved_oracle_rtm(case, shots, vps, datas, _sb(sb))
+    assert np.abs(want_img).max() > 0
+    assert np.array_equal(ill_b, want_ill)
+    assert np.array_equal(img_b, want_img)
+
+
 def test_shot_pairs_need_an_even_batch(fl):
     case = Case(nx=24, ny=24, nz=20, nt=20, amp=0.0)
     c = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=3)

@@ -135,6 +135,41 @@ def test_colocated_receivers_bit_exact(fl, fwi, two_step):
     assert np.array_equal(g_img, image)
 
 
+@pytest.mark.parametrize("tuning", [None, (4, 2, 3), (16, 1, 2)])
+def test_source_and_receivers_on_the_updated_edge(fl, tuning):
+    """The source on the first updated cell (4, 4, 4) and receivers on the
+    corners and faces of the updated region (the outermost cells a step
+    writes, next to the 4-cell rims): seismogram, image and illumination
+    bit for bit against the oracle, with z-chunk boundaries inside the
+    grid."""
+    case = Case(nt=60)
+    hi = (case.nxx - 5, case.nyy - 5, case.nzz - 5)
+    case.src = (4, 4, 4)
+    rec = [(4, 4, 4), (hi[0], hi[1], hi[2]), (4, hi[1], 4), (hi[0], 4, hi[2]), (4, 4, hi[2]), (hi[0], hi[1], 4),
+           (case.nxx // 2, 4, case.nzz // 2), (4, case.nyy // 2, case.nzz // 2), (case.nxx // 2, case.nyy // 2, 4),
+           (case.nxx // 2, case.nyy // 2, hi[2]), (5, 4, 4), (4, 5, 4), (4, 4, 5)]
+    case.rec = np.array(rec, np.int32)
+    P, Pold, seis = _oracle_forward(case)
+    rng = np.random.default_rng(9)
+    data = (seis + rng.normal(0, 1e-3, seis.shape)).astype(np.float32)
+    S, So, R, Ro, image, illum = O.backward(case.vp, case.dh, case.dt, case.nt, data, case.rec, P, Pold,
+                                            tlag=case.tlag)
+    with case.context(fl.Context) as c:
+        if tuning is not None:
+            c.set_tuning(fl.KIND_FWD_RBC, *tuning)
+            c.set_tuning(fl.KIND_BWD_RTM, *tuning)
+        c.forward(fl.BOUNDARY_RBC)
+        g_seis = c.seismogram()
+        c.set_backward_data(data)
+        c.rtm_backward()
+        g_img = c.get_volume(fl.VOL_IMAGE)
+        g_ill = c.get_volume(fl.VOL_ILLUM)
+    assert np.abs(seis).max() > 0 and np.abs(image).max() > 0
+    assert np.array_equal(g_seis, seis)
+    assert np.array_equal(g_ill, illum)
+    assert np.array_equal(g_img, image)
+
+
 @pytest.mark.parametrize("phys", [0, 3, 7])
 @pytest.mark.parametrize("arith", ["reference", "fp32"])
 @pytest.mark.parametrize("fwi", [False, True])
