@@ -351,13 +351,15 @@ def dataset_tools(w):
     return out
 
 
-def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=True):
+def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=True, real_pre=0):
     """RTM of the first `nshots` shots of the 4x4 surface survey (each shot its
     own random boundary and its own 201x201 receiver patch clipped at the
     model edges), once as one batch and once shot after shot on the
     single-shot context; returns shots/h of both and the image difference.
     With pass_steps (profiling only) the batch runs that many steps of each
-    pass on zero data and nothing else."""
+    pass on zero data and nothing else; with real_pre as well, it runs the
+    whole forward pass and real_pre backward steps on the recorded data
+    instead, then pass_steps backward steps on those wavefields."""
     w16 = configs.gradient_401_16(interior=w.interior)
     S = nshots
     log("batched RTM: %d shots, receivers per shot %s" % (S, [len(w16.receivers[s]) for s in range(S)]))
@@ -386,6 +388,22 @@ def batched_rtm(w, single, nshots, lib, configs, tuning="", pass_steps=0, graph=
     log("batched tuning %.1f s: forward %s sb=%d spt=%d, rtm %s sb=%d spt=%d" % (
         time.perf_counter() - t0, c.tuning(0), c.shots_per_block(0), c.shots_per_thread(0), c.tuning(2),
         c.shots_per_block(2), c.shots_per_thread(2)))
+    if pass_steps and real_pre:
+        c.forward(lib.BOUNDARY_RBC)
+        for s in range(S):
+            c.set_backward_data(c.seismogram(slot=s), slot=s)
+        c.backward_steps(0, 0, real_pre)
+        c.sync()
+        log("batch: forward pass and %d backward steps done" % real_pre)
+        c.set_kernel_timing(True)
+        c.backward_steps(0, real_pre, real_pre + pass_steps)
+        ms, n = c.last_kernel_timing()
+        c.close()
+        us = ms / n * 1e3 / S
+        log("batched RTM kernel on real wavefields, steps %d-%d: %.1f us per shot-step"
+            % (real_pre, real_pre + pass_steps, us))
+        return {"shots": S, "profiling_pass_steps": pass_steps, "profiling_real_pre": real_pre,
+                "rtm_us_per_shot_step": round(us, 2)}
     if pass_steps:
         for s in range(S):
             c.set_backward_data(np.zeros((len(w16.receivers[s]), w.nt), np.float32), slot=s)
@@ -521,6 +539,9 @@ def main():
                          "stream (0)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the blob-boundary and dataset-tool timings")
+    ap.add_argument("--batch-real-steps", type=int, default=0,
+                    help="profiling only (with --pass-steps): the batch runs its whole forward pass and this many "
+                         "backward steps on the recorded data before the profiled backward steps")
     ap.add_argument("--batch-shots", type=int, default=16,
                     help="also time this many shots of the 4x4 surface survey as one batch against the "
                          "per-shot path (0: skip; 16: the full survey)")
@@ -670,7 +691,7 @@ def main():
     batched = None
     if args.batch_shots > 0:
         batched = batched_rtm(w, ctx, args.batch_shots, lib, configs, args.batch_tuning, args.pass_steps,
-                              bool(args.batch_graph))
+                              bool(args.batch_graph), args.batch_real_steps)
 
     fwi_res = None
     if args.fwi_scale != "none" and not args.pass_steps:
