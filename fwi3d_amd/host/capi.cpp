@@ -1,7 +1,8 @@
 // capi.cpp -- C entry points into the host library for tests: the host-side
 // set-up arithmetic (wavelet, dampers, parameter parsing) that feeds the
-// device library, run through the Modeling class's own methods, and the
-// data-preparation steps of dataset.hpp.
+// device library, run through the Modeling class's own methods, the
+// inversion driver's line-search step, and the data-preparation steps of
+// dataset.hpp.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -10,6 +11,7 @@
 
 #include "admin.hpp"
 #include "dataset.hpp"
+#include "inversion.hpp"
 #include "modeling.hpp"
 
 namespace {
@@ -44,6 +46,20 @@ int fwi3d_host_wavelet(int nt, float dt, float fmax, float* w) {
 // Modeling::set_b1d, the 1-D Cerjan table of set_abc_dampers (modeling.cu:96-99).
 void fwi3d_host_b1d(int nb, float abc_factor, float* b1d) {
     if (nb > 0 && b1d) SetupProbe().b1d_of(nb, abc_factor, b1d);
+}
+
+// parabola_vertex / usable_step: Inversion::optimization's step
+// (inversion.cu:291-312 and the guard around it).
+float fwi3d_host_parabola_vertex(float a0, float a1, float a2, float f0, float f1, float f2) {
+    return parabola_vertex(a0, a1, a2, f0, f1, f2);
+}
+
+float fwi3d_host_usable_step(float vertex, float a1, float a2, float f1, float f2, const float* m, const float* g,
+                             long n, float vcfl, int* fell_back) {
+    bool fb = false;
+    const float s = usable_step(vertex, a1, a2, f1, f2, m, g, n, vcfl, &fb);
+    if (fell_back) *fell_back = fb ? 1 : 0;
+    return s;
 }
 
 // catch_parameter (admin.cpp:53-88); returns the value length or -1.

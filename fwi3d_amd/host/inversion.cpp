@@ -186,6 +186,34 @@ double Inversion::linesearch(float alpha) {
     return sum_res;
 }
 
+// inversion.cu:291-312 (num, den and the vertex as the reference writes them)
+float parabola_vertex(float a0, float a1, float a2, float f0, float f1, float f2) {
+    const float num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2;
+    const float den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2;
+    return 0.5f * (num / den);
+}
+
+// A vertex the reference would take but whose model the library cannot run:
+// a non-finite step (three collinear misfits, den = 0), or a step that
+// drives some slowness squared to zero or below (NaN velocities in the
+// reference) or a velocity past the stability limit.
+float usable_step(float vertex, float a1, float a2, float f1, float f2, const float* m, const float* g, long n,
+                  float vcfl, bool* fell_back) {
+    auto usable = [&](float a) {
+        if (!std::isfinite(a)) return false;
+        for (long q = 0; q < n; q++) {
+            const float mm = m[q] - a * g[q];
+            if (!(mm > 0.0f) || !(1.0f / sqrtf(mm) <= vcfl)) return false;
+        }
+        return true;
+    };
+    if (fell_back) *fell_back = false;
+    if (usable(vertex)) return vertex;
+    if (fell_back) *fell_back = true;
+    const float better = f1 < f2 ? a1 : a2;
+    return usable(better) ? better : 0.0f;
+}
+
 // inversion.cu:291-312: parabola through (0, f0), (a1, f1), (a2, f2).
 void Inversion::optimization() {
     const float dm = (1.0f / (vmin * vmin)) - (1.0f / (vmax * vmax));
@@ -195,32 +223,17 @@ void Inversion::optimization() {
     const float f1 = (float)std::sqrt(linesearch(a1));
     stage_info = "Optimization via parabolic linesearch: final modeling";
     const float f2 = (float)std::sqrt(linesearch(a2));
-    const float num = (a1 * a1 - a2 * a2) * f0 + (a2 * a2 - a0 * a0) * f1 + (a0 * a0 - a1 * a1) * f2;
-    const float den = (a1 - a2) * f0 + (a2 - a0) * f1 + (a0 - a1) * f2;
-    step = 0.5f * (num / den);
+    step = parabola_vertex(a0, a1, a2, f0, f1, f2);
     // %.9g prints every float exactly (it reads back to the same value)
     char line[256];
     std::snprintf(line, sizeof line, "Line search: a1 = %.9g, a2 = %.9g, f(0) = %.9g, f(a1) = %.9g, f(a2) = %.9g, "
                   "step = %.9g\n", a1, a2, f0, f1, f2, step);
     std::cout << line;
-    // A vertex the reference would take but whose model the library cannot
-    // run: a non-finite step (three collinear misfits, den = 0), or a step
-    // that drives some slowness squared to zero or below (NaN velocities in
-    // the reference) or a velocity past the stability limit.  The better of
-    // the two trial steps (or no step) is taken instead; otherwise the
-    // reference's vertex stands.
-    const float vcfl = cfl_velocity();
-    auto usable = [&](float a) {
-        if (!std::isfinite(a)) return false;
-        for (long q = 0; q < nPoints; q++) {
-            const float mm = m[q] - a * gradient[q];
-            if (!(mm > 0.0f) || !(1.0f / sqrtf(mm) <= vcfl)) return false;
-        }
-        return true;
-    };
-    if (!usable(step)) {
-        const float better = f1 < f2 ? a1 : a2;
-        step = usable(better) ? better : 0.0f;
+    // the reference's vertex stands unless its model cannot run; then the
+    // better of the two trial steps (or no step) is taken
+    bool fell_back = false;
+    step = usable_step(step, a1, a2, f1, f2, m.data(), gradient.data(), nPoints, cfl_velocity(), &fell_back);
+    if (fell_back) {
         std::snprintf(line, sizeof line,
                       "Line search: the vertex leaves the valid model range; step = %.9g (the better trial step)\n",
                       step);

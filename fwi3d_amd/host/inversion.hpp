@@ -14,6 +14,18 @@
 
 #include "modeling.hpp"
 
+// The parabolic line search's vertex (inversion.cu:291-312): the minimum of
+// the parabola through (a0, f0), (a1, f1), (a2, f2), in float as the
+// reference computes it (non-finite when the three misfits are collinear).
+float parabola_vertex(float a0, float a1, float a2, float f0, float f1, float f2);
+
+// The step the driver takes: the vertex when the model m - vertex * g keeps
+// every slowness squared positive and every velocity 1/sqrt(m) within vcfl;
+// otherwise the better trial step (a1 if f1 < f2, else a2) when its model
+// does; otherwise 0.  *fell_back says whether the vertex was replaced.
+float usable_step(float vertex, float a1, float a2, float f1, float f2, const float* m, const float* g, long n,
+                  float vcfl, bool* fell_back);
+
 class Inversion : public Modeling {
 private:
     float step = 0.0f;

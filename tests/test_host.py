@@ -24,6 +24,11 @@ def host():
     L.fwi3d_host_b1d.argtypes = [ctypes.c_int, ctypes.c_float, f32]
     L.fwi3d_host_catch_parameter.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     L.fwi3d_host_catch_parameter.restype = ctypes.c_int
+    L.fwi3d_host_parabola_vertex.argtypes = [ctypes.c_float] * 6
+    L.fwi3d_host_parabola_vertex.restype = ctypes.c_float
+    L.fwi3d_host_usable_step.argtypes = [ctypes.c_float] * 5 + [f32, f32, ctypes.c_long, ctypes.c_float,
+                                                                ctypes.POINTER(ctypes.c_int)]
+    L.fwi3d_host_usable_step.restype = ctypes.c_float
     return L
 
 
@@ -84,3 +89,61 @@ def test_configs_write_roundtrip(tmp_path):
     # nb = (int)(length/dh) + 1 round-trips
     length = float([l for l in text.splitlines() if l.startswith("abc_length")][0].split("=")[1].split("#")[0])
     assert int(np.float32(length) / np.float32(w.dh)) + 1 == w.nb
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def test_line_search_vertex_matches_oracle(host):
+    """Inversion::optimization's vertex (inversion.cu:291-312) equals the
+    oracle's parabola_step bit for bit, on the driver's trial steps
+    (0, 0.02 dm, 0.05 dm) and random misfits."""
+    rng = np.random.default_rng(17)
+    vmin, vmax = np.float32(1500.0), np.float32(3500.0)
+    dm = np.float32(1.0) / (vmin * vmin) - np.float32(1.0) / (vmax * vmax)
+    a0, a1, a2 = np.float32(0.0) * dm, np.float32(0.02) * dm, np.float32(0.05) * dm
+    for _ in range(200):
+        f0, f1, f2 = (_f32(v) for v in rng.uniform(1e3, 1e5, 3))
+        got = host.fwi3d_host_parabola_vertex(a0, a1, a2, f0, f1, f2)
+        want = O.parabola_step(a0, a1, a2, f0, f1, f2)
+        assert np.float32(got).view(np.int32) == np.float32(want).view(np.int32)
+
+
+def test_line_search_guard(host):
+    """The step the driver takes: the vertex when its model can run, else the
+    better trial step, else none -- three collinear misfits (den = 0, a
+    non-finite vertex), a vertex that drives a slowness squared below zero,
+    one past the stability limit, and trial steps that fail too."""
+    n = 1000
+    rng = np.random.default_rng(5)
+    v = rng.uniform(1500.0, 3500.0, n).astype(np.float32)
+    m = (np.float32(1.0) / (v * v)).astype(np.float32)
+    g = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    g[0] = np.float32(1.0)  # the cell the positivity test trips on
+    vcfl = np.float32(4500.0)
+    a1, a2 = _f32(2e-9), _f32(5e-9)
+    fb = ctypes.c_int(-1)
+
+    def step(vertex, f1, f2, vc=vcfl):
+        s = host.fwi3d_host_usable_step(vertex, a1, a2, f1, f2, m, g, n, vc, ctypes.byref(fb))
+        return s, fb.value
+
+    # a usable vertex stands, and its model is the oracle's model_step
+    vertex = _f32(3e-9)
+    assert step(vertex, 10.0, 20.0) == (vertex, 0)
+    mo, vo = O.model_step(m, g, vertex)
+    assert np.all(mo > 0) and np.all(vo <= vcfl)
+    # collinear misfits: den = 0, the vertex is not finite
+    nan_vertex = host.fwi3d_host_parabola_vertex(0.0, a1, a2, 7.0, 7.0, 7.0)
+    assert not np.isfinite(nan_vertex)
+    assert step(nan_vertex, 7.0, 7.0) == (a2, 1)   # f1 < f2 is false: a2
+    assert step(float("inf"), 5.0, 7.0) == (a1, 1)
+    # a vertex past the largest slowness squared in the gradient direction
+    huge = _f32(float(m.max()) * 2.0)
+    assert step(huge, 5.0, 7.0) == (a1, 1)
+    # a vertex that lowers a slowness squared below 1/vcfl^2 (velocity past the limit)
+    past = _f32(float(m[0] - np.float32(1.0) / (vcfl * vcfl)) * 1.01)
+    assert step(past, 9.0, 3.0) == (a2, 1)
+    # neither the vertex nor the trial steps can run: no step
+    assert step(huge, 5.0, 7.0, vc=_f32(1000.0)) == (0.0, 1)
