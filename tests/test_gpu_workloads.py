@@ -1,19 +1,19 @@
-"""Parity on the benchmark configurations' own workloads (run with -m gpu).
+"""Parity on the benchmark workloads themselves (run with -m gpu).
 
-  config 0  two-layer 101^3 forward modeling (nb 50, 1000 samples): the full
-            1000 x 6561 seismogram, GPU FP32 against the FP64 oracle
-            (reported error; the FP32 bit-exactness is test_gpu_parity's)
-  config 1  single-shot forward + RTM at 401 x 401 x 201 (nb 50): forward
-            steps [0, 160), which cover the source window (tlag 74), and RTM
-            backward steps [0, 64) from that state, fields, seismogram rows,
-            image and illumination bit for bit against the oracle
-  config 2  the 16-shot 4 x 4 survey with clipped receiver patches as one
-            autotuned batch (graph-captured time loop) on a shrunken
-            gradient model: seismograms and the stacked image / illumination
-            bit for bit against the oracle run in the batch's summation order
-  config 3  one inversion_main iteration on a shrunken config-3 problem
-            (16 shots, smoothed initial model): misfit, gradient, line-search
-            misfits, parabola step and updated model against the oracle
+  two-layer 101^3 forward modeling (nb 50, 1000 samples): the full
+      1000 x 6561 seismogram, GPU FP32 against the FP64 oracle (reported
+      error; the FP32 bit-exactness is test_gpu_parity's)
+  single-shot forward + RTM at 401 x 401 x 201 (nb 50): forward steps
+      [0, 160), which cover the source window (tlag 74), and RTM backward
+      steps [0, 64) from that state, fields, seismogram rows, image and
+      illumination bit for bit against the oracle
+  the 16-shot 4 x 4 survey with clipped receiver patches as one batch
+      (autotuned, or shot pairs; graph-captured time loop) on a shrunken
+      gradient model: seismograms and the stacked image / illumination bit
+      for bit against the oracle run in the batch's summation order
+  one inversion_main iteration of the 16-shot FWI problem, shrunken
+      (smoothed initial model): misfit, gradient, line-search misfits,
+      parabola step and updated model against the oracle
 """
 import os
 import re
@@ -46,7 +46,7 @@ def _context(fl, w, vp_pad=None, nshots=1):
     return c
 
 
-def test_config0_seismogram_fp32_vs_fp64(fl):
+def test_two_layer_101_seismogram_fp32_vs_fp64(fl):
     """The FP32 paths' error against an FP64 solution of the same scheme
     (double fields, the same float velocity model) on the full two-layer
     101^3 run: rel-L2 of the 1000 x 6561 seismogram <= 1e-5 in both
@@ -64,12 +64,12 @@ def test_config0_seismogram_fp32_vs_fp64(fl):
             c.set_receivers(w.rec)
             c.forward(fl.BOUNDARY_RBC)
             errs[arith] = rel_l2(c.seismogram(), want)
-    print("config 0 seismogram rel-L2 vs FP64: reference %.3g, fp32 %.3g" % (errs[0], errs[1]))
+    print("two-layer 101^3 seismogram rel-L2 vs FP64: reference %.3g, fp32 %.3g" % (errs[0], errs[1]))
     assert np.abs(want).max() > 0
     assert errs[fl.ARITH_REFERENCE] <= FP64_BOUND and errs[fl.ARITH_FP32] <= FP64_BOUND, errs
 
 
-def test_config1_forward_and_rtm_segments_bit_exact(fl):
+def test_gradient_401_forward_and_rtm_segments_bit_exact(fl):
     """401 x 401 x 201 + 50-cell random boundary, 40401 receivers: forward
     steps [0, 160) and RTM backward steps [0, 64) from the forward state, on
     seeded data, against the oracle bit for bit."""
@@ -107,7 +107,7 @@ def test_config1_forward_and_rtm_segments_bit_exact(fl):
 
 
 @pytest.mark.parametrize("shape", ["autotuned", "pairs"])
-def test_config2_survey_batch_bit_exact_vs_oracle(fl, shape):
+def test_16_shot_survey_batch_bit_exact_vs_oracle(fl, shape):
     """The 16-shot 4 x 4 survey (receiver patches clipped at the edges: 1024
     to 1936 receivers per shot) on a 101 x 101 x 51 gradient model with a
     20-cell boundary, autotuned tile and shots per block (or shot pairs on
@@ -133,7 +133,7 @@ def test_config2_survey_batch_bit_exact_vs_oracle(fl, shape):
         c.autotune(fl.KIND_BWD_RTM, 3)
     sb = c.shots_per_block(fl.KIND_BWD_RTM)
     assert shape != "pairs" or (sb == 2 and c.shots_per_thread(fl.KIND_BWD_RTM) == 2)
-    print("config 2 tuning: forward %s sb=%d, rtm %s sb=%d"
+    print("16-shot survey tuning: forward %s sb=%d, rtm %s sb=%d"
           % (c.tuning(0), c.shots_per_block(0), c.tuning(2), sb))
     c.set_graph(True)
     c.forward(fl.BOUNDARY_RBC)
@@ -172,7 +172,7 @@ def _run(exe, params):
 
 
 @pytest.mark.parametrize("case", ["vertex", "guarded"])
-def test_config3_inversion_iteration_vs_oracle(fl, tmp_path, case):
+def test_16_shot_inversion_iteration_vs_oracle(fl, tmp_path, case):
     """One inversion_main iteration (16 shots, batch_shots = 1, reference-style
     blob boundary) on a shrunken gradient model from a smoothed initial model:
     the first misfit, the exported normalised gradient, the two line-search
