@@ -62,7 +62,7 @@ void Modeling::set_build_parameters() {
     if (a == "reference") arith = FWI_ARITH_REFERENCE;
     else if (a == "fp32") arith = FWI_ARITH_FP32;
     else throw std::invalid_argument("arithmetic must be 'reference' or 'fp32'");
-    const std::string at = catch_parameter_or("autotune", parameters, "off");
+    const std::string at = catch_parameter_or("autotune", parameters, "on");
     if (at == "off") autotune = 0;
     else if (at == "on") autotune = 1;
     else if (at == "all") autotune = 2;

@@ -63,11 +63,12 @@ protected:
     bool rbc_blobs = true;  // rbc_style
     int arith = FWI_ARITH_REFERENCE;
     int physics = 0;  // FWI_PHYS_* bits for the backward passes
-    // autotune key: 0 off (default launch shapes), 1 forward passes timed,
-    // backward passes on fixed one-shot 64 x 16 tiles (the summation order
-    // of the default shape), 2 every pass kind timed, shots per block or
+    // autotune key: 0 off (the library's default launch shapes), 1 (the
+    // default) forward passes timed, backward passes on fixed one-shot
+    // 64 x 16 tiles (the summation order of the default shape: results
+    // equal those of 0), 2 every pass kind timed, shots per block or
     // thread included
-    int autotune = 0;
+    int autotune = 1;
     std::string autotune_cache;     // optional cache file of the choices
     unsigned tuned_kinds = 0;       // bit (kind + 4 * context): done
 
