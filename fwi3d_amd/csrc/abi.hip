@@ -937,12 +937,26 @@ int fwi_create_batch(const fwi_grid* grid, int nshots, int device, fwi_ctx** out
         return bail(fail(c, FWI_ERR_HIP, "hipMemset failed"));
     if ((rc = sync_geometry(c))) return bail(rc);
     const bool big = c->L.vol * 4 >= (1L << 31);
-    for (auto& t : c->tune) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    for (int k = 0; k < 4; ++k) {
+        Tuning& t = c->tune[k];
         t = Tuning{};
         if (big) {  // the per-plane-addressed instances
             t.big = 1;
             t.by = 16;
             t.rpt = 1;
+        } else if (nshots == 1) {
+            // one-shot contexts: 64 x 32 tiles for the forward passes and
+            // 64 x 16 for the backward ones (16 waves per block), with the
+            // fewest z-chunks that give about one block per CU -- the shapes
+            // the autotuner picks on large grids (batched contexts keep the
+            // smaller tiles that their shots-per-block instances take)
+            t.by = 16;
+            t.rpt = k < 2 ? 2 : 1;
+            const long tiles = (long)((c->L.nx + 63) / 64) * ((c->L.ny - 8 + t.by * t.rpt - 1) / (t.by * t.rpt));
+            while (tiles * t.zchunks < cus * 15L / 16 && t.zchunks < 16 && t.zchunks < g.nzz - 8) ++t.zchunks;
         }
     }
     *out = c;

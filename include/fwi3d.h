@@ -78,7 +78,10 @@ typedef struct fwi_ctx fwi_ctx;
  * illumination and gradient are allocated on first use.  Padded volumes of
  * 2 GiB and more run step-kernel instances that address each z-plane with
  * its own 64-bit base (one shot per block, image and illumination updated
- * every step); one padded x-y plane must stay below 2 GiB. */
+ * every step); one padded x-y plane must stay below 2 GiB.  Default launch
+ * shapes: 64 x 32 tiles for the forward passes and 64 x 16 for the backward
+ * ones, with the fewest z-chunks that give about one block per CU
+ * (fwi_autotune times the alternatives). */
 int fwi_create(const fwi_grid* grid, int device, fwi_ctx** out);
 int fwi_destroy(fwi_ctx* ctx);
 const char* fwi_last_error(const fwi_ctx* ctx);
