@@ -431,6 +431,25 @@ def test_errors(fl):
         assert e.value.code == -4
 
 
+def test_default_launch_shapes(fl):
+    """A one-shot context starts from 64 x 32 forward and 64 x 16 backward
+    tiles with the fewest z-chunks that give about one block per CU (the
+    autotuner's picks at 401 x 401 x 201: (16, 2, 2) and (16, 1, 1)); a
+    small grid takes more z-chunks, up to 16; a batched context keeps the
+    64 x 8 tiles its shots-per-block instances take."""
+    big = fl.Context(501, 501, 301, 50, 10.0, 8e-4, 10, 0)
+    assert [big.tuning(k) for k in range(4)] == [(16, 2, 2), (16, 2, 2), (16, 1, 1), (16, 1, 1)]
+    big.close()
+    case = Case()
+    small = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag)
+    assert small.tuning(0) == (16, 2, 16) and small.tuning(2) == (16, 1, 16)
+    small.close()
+    batch = fl.Context(case.nxx, case.nyy, case.nzz, case.nb, case.dh, case.dt, case.nt, case.tlag, nshots=4)
+    assert all(batch.tuning(k) == (4, 2, 1) for k in range(4))
+    batch.set_shots_per_block(2, 4)  # the batched instances take the default tile
+    batch.close()
+
+
 @pytest.mark.slow
 def test_two_layer_101_cube_seismogram(fl):
     """The 101^3 two-layer workload (nb = 50, 1000 steps, 81x81 receivers):
